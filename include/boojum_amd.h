@@ -1,0 +1,155 @@
+/*
+ * boojum_amd.h -- C-ABI of the MI355X-native Goldilocks NTT / coset-LDE library
+ * (libboojum_amd.so, built from zksync-crypto-gpu_amd/csrc for gfx950).
+ *
+ * It replaces, for the NTT/LDE path only:
+ *   - era_cudart / cudart-sys (crates/cudart, crates/cudart-sys): the runtime subset the
+ *     path rides on (streams, events, memory, graphs), as thin HIP wrappers;
+ *   - boojum_cuda::context (crates/boojum-cuda/src/context.rs): twiddle / coset tables;
+ *   - boojum_cuda::ntt (crates/boojum-cuda/src/ntt.rs): the batch NTT entry points;
+ *   - boojum_cuda::ops_complex::bit_reverse* (crates/boojum-cuda/src/ops_complex.rs:262-331).
+ *
+ * Conventions
+ *   - Every function returns an int status: 0 (BA_SUCCESS) or a HIP error code; argument
+ *     preconditions that the reference enforces with assert! return BA_ERROR_INVALID_VALUE
+ *     (hipErrorInvalidValue) instead of panicking.  Nothing here aborts the process.
+ *   - Field elements are uint64_t.  Inputs may hold any u64 (read mod p); outputs are
+ *     canonical (< p).  Matrices are column-major: column k starts at ptr + k * stride.
+ *   - Device pointers are plain pointers; streams/events/graphs are opaque handles (a null
+ *     stream is the default stream).  All compute calls are asynchronous on their stream and
+ *     borrow their buffers: the caller keeps them alive until the stream is synchronised.
+ */
+#ifndef BOOJUM_AMD_H
+#define BOOJUM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BA_SUCCESS 0
+#define BA_ERROR_INVALID_VALUE 1         /* hipErrorInvalidValue */
+#define BA_ERROR_OUT_OF_MEMORY 2         /* hipErrorOutOfMemory */
+#define BA_ERROR_NOT_INITIALIZED 3       /* hipErrorNotInitialized: no live context */
+
+typedef void *ba_stream_t;     /* hipStream_t */
+typedef void *ba_event_t;      /* hipEvent_t */
+typedef void *ba_graph_exec_t; /* hipGraphExec_t */
+typedef struct ba_context *ba_context_t;
+
+/* ---------------------------------------------------------------- runtime subset
+ * Replaces the cudart calls the path uses (crates/cudart/src/{device,memory,stream,event,
+ * error}.rs); each is a 1:1 HIP call with its error code returned as int. */
+int ba_get_device_count(int *count);
+int ba_set_device(int device);
+int ba_get_device(int *device);
+int ba_device_synchronize(void);
+int ba_mem_get_info(size_t *free_bytes, size_t *total_bytes);
+/* hipDeviceAttribute_t value; e.g. 16 = multiprocessor count */
+int ba_device_get_attribute(int *value, int attribute, int device);
+int ba_malloc(void **ptr, size_t bytes);
+int ba_free(void *ptr);
+int ba_malloc_host(void **ptr, size_t bytes); /* pinned */
+int ba_free_host(void *ptr);
+/* kind: 0 host->host, 1 host->device, 2 device->host, 3 device->device, 4 default */
+int ba_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int ba_memcpy_async(void *dst, const void *src, size_t bytes, int kind, ba_stream_t stream);
+int ba_memset_async(void *dst, int value, size_t bytes, ba_stream_t stream);
+int ba_stream_create(ba_stream_t *stream); /* non-blocking, default priority */
+int ba_stream_destroy(ba_stream_t stream);
+int ba_stream_synchronize(ba_stream_t stream);
+int ba_stream_wait_event(ba_stream_t stream, ba_event_t event);
+int ba_event_create(ba_event_t *event);
+int ba_event_destroy(ba_event_t event);
+int ba_event_record(ba_event_t event, ba_stream_t stream);
+int ba_event_synchronize(ba_event_t event);
+int ba_event_query(ba_event_t event); /* 0 done, hipErrorNotReady (600) pending */
+int ba_event_elapsed_ms(float *ms, ba_event_t start, ba_event_t end);
+int ba_get_last_error(void);
+const char *ba_error_name(int status);
+int ba_stream_begin_capture(ba_stream_t stream);
+int ba_stream_end_capture(ba_stream_t stream, ba_graph_exec_t *exec);
+int ba_graph_launch(ba_graph_exec_t exec, ba_stream_t stream);
+int ba_graph_destroy(ba_graph_exec_t exec);
+
+/* ---------------------------------------------------------------- context
+ * boojum_cuda::context::Context::create / destroy (crates/boojum-cuda/src/context.rs:171-296).
+ * Builds the power tables of omega_{2^30}, g = 7 and g^-1 split into fine/coarse halves
+ * (coarse sizes 2^w_coarse_log and 2^g_coarse_log, each <= 30) plus 2^-k for k <= 30, and
+ * keeps them resident in HBM.  The most recently created live context of the current device
+ * is the one the context-free entry points below use, as the reference's process-global
+ * tables are (context.rs:55-60). */
+int ba_context_create(uint32_t powers_of_w_coarse_log_count, uint32_t powers_of_g_coarse_log_count,
+                      ba_context_t *ctx);
+int ba_context_destroy(ba_context_t ctx);
+#define BA_OMEGA_LOG_ORDER 30u
+
+/* ---------------------------------------------------------------- NTT
+ * boojum_cuda::ntt::batch_ntt_internal (crates/boojum-cuda/src/ntt.rs:179-291).
+ * num_ntts columns of 2^log_n elements; input column k at inputs + k*stride_in, output at
+ * outputs + k*stride_out (inputs == outputs: in place).
+ *   bitrev_inputs=0, inverse=0: out[bitrev(k)] = sum_j s^j x_j w^(jk)       (natural -> bitrev)
+ *   bitrev_inputs=1, inverse=0: input bit-reversed, output natural
+ *   bitrev_inputs=0, inverse=1: out[bitrev(j)] = n^-1 s^-j sum_k X_k w^(-jk)
+ *   bitrev_inputs=1, inverse=1: input bit-reversed, output natural
+ * with w = omega_{2^log_n} and s = 1 if log_extension_degree == 0 else
+ * 7 * omega_{2^(log_n + log_extension_degree)}^coset_index.
+ * Preconditions (ntt.rs:192-200): 1 <= log_n, log_n + log_extension_degree <= 30,
+ * 2^log_n <= strides, coset_index < 2^log_extension_degree when nonzero. */
+int ba_batch_ntt_internal(const uint64_t *inputs, uint64_t *outputs, uint32_t log_n,
+                          uint32_t num_ntts, uint32_t stride_between_input_arrays,
+                          uint32_t stride_between_output_arrays, int bitrev_inputs, int inverse,
+                          uint32_t log_extension_degree, uint32_t coset_index, ba_stream_t stream);
+
+/* boojum_cuda::ntt::batch_ntt_out_of_place (ntt.rs:294-335): DeviceSlices passed as
+ * (pointer, length in elements); checks offset + n <= stride, lengths >= num_ntts * stride
+ * and that the (offset) pointers differ. */
+int ba_batch_ntt_out_of_place(const uint64_t *inputs_matrix, size_t inputs_len,
+                              uint64_t *outputs_matrix, size_t outputs_len, uint32_t log_n,
+                              uint32_t num_ntts, uint32_t inputs_offset, uint32_t outputs_offset,
+                              uint32_t stride_between_input_arrays,
+                              uint32_t stride_between_output_arrays, int bitrev_inputs,
+                              int inverse, uint32_t log_extension_degree, uint32_t coset_index,
+                              ba_stream_t stream);
+
+/* boojum_cuda::ntt::batch_ntt_in_place (ntt.rs:338-369). */
+int ba_batch_ntt_in_place(uint64_t *inputs_matrix, size_t inputs_len, uint32_t log_n,
+                          uint32_t num_ntts, uint32_t inputs_offset,
+                          uint32_t stride_between_input_arrays, int bitrev_inputs, int inverse,
+                          uint32_t log_extension_degree, uint32_t coset_index, ba_stream_t stream);
+
+/* Same as ba_batch_ntt_internal with an explicit context (several may be live). */
+int ba_batch_ntt_ctx(ba_context_t ctx, const uint64_t *inputs, uint64_t *outputs, uint32_t log_n,
+                     uint32_t num_ntts, uint64_t stride_in, uint64_t stride_out, int bitrev_inputs,
+                     int inverse, uint32_t log_extension_degree, uint32_t coset_index,
+                     ba_stream_t stream);
+
+/* ---------------------------------------------------------------- bit reverse
+ * boojum_cuda::ops_complex::bit_reverse / bit_reverse_in_place (ops_complex.rs:262-331):
+ * dst[k][bitrev(i)] = src[k][i] for cols columns of 2^log_rows rows (src == dst: in place). */
+int ba_bit_reverse(const uint64_t *src, uint64_t src_stride, uint64_t *dst, uint64_t dst_stride,
+                   uint32_t log_rows, uint32_t cols, ba_stream_t stream);
+int ba_bit_reverse_in_place(uint64_t *values, uint64_t stride, uint32_t log_rows, uint32_t cols,
+                            ba_stream_t stream);
+
+/* ---------------------------------------------------------------- helpers
+ * Seeded uniform field elements on the device: element i of column k is the SplitMix64
+ * finaliser of s_k + (i+1) * 0x9E3779B97F4A7C15 with s_k = seed + k * 0xD1B54A32D192ED03,
+ * re-mixed while >= p.  No reference
+ * counterpart (its benches draw unseeded host rng, benches/ntt.rs:70); used to build large
+ * synthetic inputs in HBM without a host round trip. */
+int ba_fill_uniform(uint64_t *dst, uint64_t stride, uint64_t rows, uint32_t cols, uint64_t seed,
+                    ba_stream_t stream);
+
+/* Element-wise field operation on the device (op: 0 add, 1 sub, 2 mul, 3 canonicalise,
+ * 4 inverse), out[i] = op(a[i], b[i]).  Exposes the kernels' field arithmetic for parity
+ * tests (goldilocks.cuh semantics, crates/boojum-cuda/native/goldilocks.cuh:90-317). */
+int ba_field_op(int op, const uint64_t *a, const uint64_t *b, uint64_t *out, size_t n,
+                ba_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

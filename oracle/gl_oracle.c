@@ -1,0 +1,325 @@
+/*
+ * gl_oracle.c -- CPU oracle for the Goldilocks NTT / coset-LDE path.
+ *
+ * TEST INFRASTRUCTURE ONLY: linked by tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg, as the checker or the timed CPU baseline.  Never on the product path.
+ *
+ * Restates, from the field's mathematics, boojum's CPU FFT (crates.io boojum =0.32.10,
+ * Cargo.toml:32; absent from the reference tree) exactly as the reference's own test
+ * calls it (crates/boojum-cuda/src/ntt.rs:391-611).  Parity of each function is pinned by
+ * tests/test_oracle.py (closed-form roots, an independent big-integer DFT, transform
+ * invariants and direct polynomial evaluation on coset points).
+ */
+#include "gl_oracle.h"
+
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef unsigned __int128 u128;
+
+
+uint64_t gl_reduce(uint64_t x) { return x >= GL_P ? x - GL_P : x; }
+
+uint64_t gl_add(uint64_t a, uint64_t b) {
+  u128 s = (u128)a + b;
+  return (uint64_t)(s % GL_P);
+}
+
+uint64_t gl_sub(uint64_t a, uint64_t b) { return a >= b ? a - b : a + (GL_P - b); }
+
+uint64_t gl_mul(uint64_t a, uint64_t b) { return (uint64_t)(((u128)a * b) % GL_P); }
+
+uint64_t gl_pow(uint64_t a, uint64_t e) {
+  uint64_t r = 1, x = gl_reduce(a);
+  while (e) {
+    if (e & 1) r = gl_mul(r, x);
+    x = gl_mul(x, x);
+    e >>= 1;
+  }
+  return r;
+}
+
+uint64_t gl_inv(uint64_t a) { return gl_pow(a, GL_P - 2); }
+
+uint64_t gl_generator(void) { return 7; }
+
+/* domain_generator_for_size: the 2^32-th root of unity 7^((p-1)/2^32) squared
+ * (32 - log_size) times (crates/boojum-cuda/src/context.rs:179-198 call site). */
+uint64_t gl_omega(unsigned log_size) {
+  uint64_t w = gl_pow(7, (GL_P - 1) >> 32);
+  for (unsigned i = log_size; i < 32; i++) w = gl_mul(w, w);
+  return w;
+}
+
+size_t gl_bitrev(size_t i, unsigned bits) {
+  size_t r = 0;
+  for (unsigned b = 0; b < bits; b++) r |= ((i >> b) & 1) << (bits - 1 - b);
+  return r;
+}
+
+void gl_bitreverse_enumeration(uint64_t *v, unsigned log_n) {
+  size_t n = (size_t)1 << log_n;
+  for (size_t i = 0; i < n; i++) {
+    size_t j = gl_bitrev(i, log_n);
+    if (i < j) {
+      uint64_t t = v[i];
+      v[i] = v[j];
+      v[j] = t;
+    }
+  }
+}
+
+void gl_distribute_powers(uint64_t *v, size_t n, uint64_t base) {
+  uint64_t acc = 1;
+  for (size_t i = 0; i < n; i++) {
+    v[i] = gl_mul(gl_reduce(v[i]), acc);
+    acc = gl_mul(acc, base);
+  }
+}
+
+/* precompute_twiddles_for_fft (ntt.rs:404-416 call site): the first n/2 powers of
+ * omega_n (or its inverse), in bit-reversed order over log_n - 1 bits. */
+void gl_precompute_twiddles(uint64_t *tw, unsigned log_n, int inverse) {
+  if (log_n == 0) return;
+  size_t half = (size_t)1 << (log_n - 1);
+  uint64_t w = gl_omega(log_n);
+  if (inverse) w = gl_inv(w);
+  uint64_t acc = 1;
+  for (size_t i = 0; i < half; i++) {
+    tw[i] = acc;
+    acc = gl_mul(acc, w);
+  }
+  gl_bitreverse_enumeration(tw, log_n - 1);
+}
+
+/* Radix-2 Cooley-Tukey, natural order in, bit-reversed order out: at stage s the 2^s
+ * groups of size n/2^s use twiddle tw[group] (bit-reversed table). */
+void gl_serial_ct_n2b(uint64_t *a, unsigned log_n, const uint64_t *tw) {
+  size_t n = (size_t)1 << log_n;
+  for (size_t i = 0; i < n; i++) a[i] = gl_reduce(a[i]);
+  size_t groups = 1, dist = n >> 1;
+  while (groups < n) {
+    for (size_t k = 0; k < groups; k++) {
+      size_t start = k * dist * 2;
+      uint64_t w = tw[k];
+      for (size_t j = start; j < start + dist; j++) {
+        uint64_t u = a[j];
+        uint64_t v = gl_mul(a[j + dist], w);
+        a[j] = gl_add(u, v);
+        a[j + dist] = gl_sub(u, v);
+      }
+    }
+    groups <<= 1;
+    dist >>= 1;
+  }
+}
+
+void gl_fft_natural_to_bitreversed(uint64_t *v, unsigned log_n, uint64_t coset, const uint64_t *tw) {
+  size_t n = (size_t)1 << log_n;
+  if (coset != 1) gl_distribute_powers(v, n, coset);
+  gl_serial_ct_n2b(v, log_n, tw);
+}
+
+void gl_ifft_natural_to_natural(uint64_t *v, unsigned log_n, uint64_t coset, const uint64_t *tw) {
+  size_t n = (size_t)1 << log_n;
+  gl_serial_ct_n2b(v, log_n, tw);
+  gl_bitreverse_enumeration(v, log_n);
+  if (coset != 1) gl_distribute_powers(v, n, gl_inv(coset));
+  if (n > 1) {
+    uint64_t n_inv = gl_inv(n % GL_P);
+    for (size_t i = 0; i < n; i++) v[i] = gl_mul(v[i], n_inv);
+  }
+}
+
+/* coset = g * omega_{n*2^e}^c when log_ext > 0 (ntt.rs:527-537) */
+uint64_t gl_coset_shift(unsigned log_n, unsigned log_ext, unsigned coset_idx) {
+  if (log_ext == 0) return 1;
+  uint64_t w = gl_omega(log_n + log_ext);
+  return gl_mul(gl_generator(), gl_pow(w, coset_idx));
+}
+
+/* ---------------------------------------------------------------- batch driver */
+
+typedef struct {
+  const uint64_t *in;
+  uint64_t *out;
+  unsigned log_n;
+  uint64_t stride_in, stride_out;
+  int bitrev_inputs, inverse;
+  uint64_t coset;
+  const uint64_t *tw;
+  unsigned col_begin, col_end;
+} ntt_job;
+
+static void *ntt_worker(void *arg) {
+  ntt_job *j = (ntt_job *)arg;
+  size_t n = (size_t)1 << j->log_n;
+  uint64_t *buf = (uint64_t *)malloc(n * sizeof(uint64_t));
+  for (unsigned c = j->col_begin; c < j->col_end; c++) {
+    const uint64_t *src = j->in + (size_t)c * j->stride_in;
+    memcpy(buf, src, n * sizeof(uint64_t));
+    /* b2n takes the transform input bit-reversed (ntt.rs:441-447, 577-583) */
+    if (j->bitrev_inputs) gl_bitreverse_enumeration(buf, j->log_n);
+    if (j->inverse) {
+      gl_ifft_natural_to_natural(buf, j->log_n, j->coset, j->tw);
+      /* n2b inverse returns bit-reversed order (ntt.rs:577-578) */
+      if (!j->bitrev_inputs) gl_bitreverse_enumeration(buf, j->log_n);
+    } else {
+      gl_fft_natural_to_bitreversed(buf, j->log_n, j->coset, j->tw);
+      /* b2n forward returns natural order (ntt.rs:581-582) */
+      if (j->bitrev_inputs) gl_bitreverse_enumeration(buf, j->log_n);
+    }
+    memcpy(j->out + (size_t)c * j->stride_out, buf, n * sizeof(uint64_t));
+  }
+  free(buf);
+  return NULL;
+}
+
+static void run_jobs(void *(*fn)(void *), void *jobs, size_t job_size, int nthreads) {
+  if (nthreads <= 1) {
+    fn(jobs);
+    return;
+  }
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+  for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, fn, (char *)jobs + (size_t)t * job_size);
+  for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+  free(th);
+}
+
+int gl_oracle_batch_ntt(const uint64_t *in, uint64_t *out, unsigned log_n, unsigned num_ntts,
+                        uint64_t stride_in, uint64_t stride_out, int bitrev_inputs, int inverse,
+                        unsigned log_ext, unsigned coset_idx, int nthreads) {
+  /* argument checks of batch_ntt_internal (ntt.rs:192-200) */
+  if (log_n < 1 || log_n + log_ext > GL_OMEGA_LOG_ORDER) return -1;
+  uint64_t n = (uint64_t)1 << log_n;
+  if (n > stride_in || n > stride_out) return -1;
+  if (coset_idx > 0 && (log_ext == 0 || coset_idx >= (1u << log_ext))) return -1;
+  if (num_ntts == 0) return 0;
+  uint64_t *tw = (uint64_t *)malloc(sizeof(uint64_t) * (n / 2));
+  gl_precompute_twiddles(tw, log_n, inverse);
+  uint64_t coset = gl_coset_shift(log_n, log_ext, coset_idx);
+  if (nthreads < 1) nthreads = 1;
+  if ((unsigned)nthreads > num_ntts) nthreads = (int)num_ntts;
+  ntt_job *jobs = (ntt_job *)calloc((size_t)nthreads, sizeof(ntt_job));
+  for (int t = 0; t < nthreads; t++) {
+    ntt_job *j = &jobs[t];
+    j->in = in;
+    j->out = out;
+    j->log_n = log_n;
+    j->stride_in = stride_in;
+    j->stride_out = stride_out;
+    j->bitrev_inputs = bitrev_inputs;
+    j->inverse = inverse;
+    j->coset = coset;
+    j->tw = tw;
+    j->col_begin = (unsigned)((uint64_t)num_ntts * t / nthreads);
+    j->col_end = (unsigned)((uint64_t)num_ntts * (t + 1) / nthreads);
+  }
+  run_jobs(ntt_worker, jobs, sizeof(ntt_job), nthreads);
+  free(jobs);
+  free(tw);
+  return 0;
+}
+
+void gl_oracle_bit_reverse(const uint64_t *src, uint64_t src_stride, uint64_t *dst,
+                           uint64_t dst_stride, unsigned log_rows, unsigned cols) {
+  size_t n = (size_t)1 << log_rows;
+  uint64_t *buf = (uint64_t *)malloc(n * sizeof(uint64_t));
+  for (unsigned c = 0; c < cols; c++) {
+    const uint64_t *s = src + (size_t)c * src_stride;
+    for (size_t i = 0; i < n; i++) buf[gl_bitrev(i, log_rows)] = s[i];
+    memcpy(dst + (size_t)c * dst_stride, buf, n * sizeof(uint64_t));
+  }
+  free(buf);
+}
+
+/* ---------------------------------------------------------------- LDE */
+
+typedef struct {
+  const uint64_t *evals;
+  uint64_t in_stride;
+  uint64_t *out;
+  uint64_t out_col_stride, out_coset_stride;
+  unsigned log_n, log_blowup;
+  const uint64_t *tw_inv, *tw_fwd;
+  unsigned col_begin, col_end;
+} lde_job;
+
+static void *lde_worker(void *arg) {
+  lde_job *j = (lde_job *)arg;
+  size_t n = (size_t)1 << j->log_n;
+  unsigned blowup = 1u << j->log_blowup;
+  uint64_t *mono = (uint64_t *)malloc(n * sizeof(uint64_t));
+  uint64_t *buf = (uint64_t *)malloc(n * sizeof(uint64_t));
+  for (unsigned k = j->col_begin; k < j->col_end; k++) {
+    /* into_monomials: n2b inverse NTT then bit reverse (storage.rs:155-169) */
+    memcpy(mono, j->evals + (size_t)k * j->in_stride, n * sizeof(uint64_t));
+    gl_ifft_natural_to_natural(mono, j->log_n, 1, j->tw_inv);
+    for (unsigned c = 0; c < blowup; c++) {
+      /* raw_batch_coset_ntt bit-reverses the coset index (primitives/ntt.rs:25) */
+      unsigned cidx = (unsigned)gl_bitrev(c, j->log_blowup);
+      uint64_t s = gl_coset_shift(j->log_n, j->log_blowup, cidx);
+      memcpy(buf, mono, n * sizeof(uint64_t));
+      gl_fft_natural_to_bitreversed(buf, j->log_n, s, j->tw_fwd);
+      memcpy(j->out + (size_t)k * j->out_col_stride + (size_t)c * j->out_coset_stride, buf,
+             n * sizeof(uint64_t));
+    }
+  }
+  free(buf);
+  free(mono);
+  return NULL;
+}
+
+int gl_oracle_lde(const uint64_t *evals, uint64_t in_stride, uint64_t *out,
+                  uint64_t out_col_stride, uint64_t out_coset_stride, unsigned log_n,
+                  unsigned num_cols, unsigned log_blowup, int nthreads) {
+  if (log_n < 1 || log_blowup < 1 || log_n + log_blowup > GL_OMEGA_LOG_ORDER) return -1;
+  if (num_cols == 0) return 0;
+  size_t n = (size_t)1 << log_n;
+  uint64_t *tw_inv = (uint64_t *)malloc(sizeof(uint64_t) * (n / 2));
+  uint64_t *tw_fwd = (uint64_t *)malloc(sizeof(uint64_t) * (n / 2));
+  gl_precompute_twiddles(tw_inv, log_n, 1);
+  gl_precompute_twiddles(tw_fwd, log_n, 0);
+  if (nthreads < 1) nthreads = 1;
+  if ((unsigned)nthreads > num_cols) nthreads = (int)num_cols;
+  lde_job *jobs = (lde_job *)calloc((size_t)nthreads, sizeof(lde_job));
+  for (int t = 0; t < nthreads; t++) {
+    lde_job *j = &jobs[t];
+    j->evals = evals;
+    j->in_stride = in_stride;
+    j->out = out;
+    j->out_col_stride = out_col_stride;
+    j->out_coset_stride = out_coset_stride;
+    j->log_n = log_n;
+    j->log_blowup = log_blowup;
+    j->tw_inv = tw_inv;
+    j->tw_fwd = tw_fwd;
+    j->col_begin = (unsigned)((uint64_t)num_cols * t / nthreads);
+    j->col_end = (unsigned)((uint64_t)num_cols * (t + 1) / nthreads);
+  }
+  run_jobs(lde_worker, jobs, sizeof(lde_job), nthreads);
+  free(jobs);
+  free(tw_fwd);
+  free(tw_inv);
+  return 0;
+}
+
+/* ---------------------------------------------------------------- inputs */
+
+/* Counter-based SplitMix64: element i is the SplitMix64 finaliser of seed + (i+1)*gamma;
+ * a value >= p is re-mixed until it is below p. */
+static uint64_t splitmix_mix(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+void gl_fill_uniform(uint64_t *v, size_t n, uint64_t seed) {
+  const uint64_t gamma = 0x9E3779B97F4A7C15ull;
+  for (size_t i = 0; i < n; i++) {
+    uint64_t x = splitmix_mix(seed + (uint64_t)(i + 1) * gamma);
+    while (x >= GL_P) x = splitmix_mix(x ^ gamma);
+    v[i] = x;
+  }
+}

@@ -1,0 +1,121 @@
+"""ctypes view of the CPU oracle (oracle/libgl_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, where it is the checker or the timed CPU baseline.  Nothing on the
+product path imports it.
+
+The C restatement follows boojum's CPU FFT as the reference test calls it
+(crates/boojum-cuda/src/ntt.rs:391-611) and shivini's LDE composition
+(crates/shivini/src/data_structures/storage.rs:154-260, primitives/ntt.rs:10-42).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+P = 0xFFFFFFFF00000001
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libgl_oracle.so")
+_lib = None
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+def _ptr(a):
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_u64p)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(_LIB_PATH)
+        u64, u32, i32 = ctypes.c_uint64, ctypes.c_uint, ctypes.c_int
+        for name, res, args in [
+            ("gl_reduce", u64, [u64]),
+            ("gl_add", u64, [u64, u64]),
+            ("gl_sub", u64, [u64, u64]),
+            ("gl_mul", u64, [u64, u64]),
+            ("gl_pow", u64, [u64, u64]),
+            ("gl_inv", u64, [u64]),
+            ("gl_omega", u64, [u32]),
+            ("gl_coset_shift", u64, [u32, u32, u32]),
+            ("gl_precompute_twiddles", None, [_u64p, u32, i32]),
+            ("gl_oracle_batch_ntt", i32, [_u64p, _u64p, u32, u32, u64, u64, i32, i32, u32, u32, i32]),
+            ("gl_oracle_bit_reverse", None, [_u64p, u64, _u64p, u64, u32, u32]),
+            ("gl_oracle_lde", i32, [_u64p, u64, _u64p, u64, u64, u32, u32, u32, i32]),
+            ("gl_fill_uniform", None, [_u64p, ctypes.c_size_t, u64]),
+        ]:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def field_op(name, *args):
+    return int(getattr(lib(), name)(*args))
+
+
+def omega(log_size):
+    return int(lib().gl_omega(log_size))
+
+
+def coset_shift(log_n, log_ext, coset_idx):
+    return int(lib().gl_coset_shift(log_n, log_ext, coset_idx))
+
+
+def twiddles(log_n, inverse):
+    tw = np.zeros(max(1, (1 << log_n) // 2), dtype=np.uint64)
+    lib().gl_precompute_twiddles(_ptr(tw), log_n, int(inverse))
+    return tw[: (1 << log_n) // 2]
+
+
+def batch_ntt(inputs, log_n, num_ntts, stride_in, stride_out, bitrev_inputs, inverse,
+              log_ext=0, coset_idx=0, out=None, nthreads=1):
+    """Out-of-place (or in place when out is inputs) batch NTT with the semantics of
+    batch_ntt_internal (crates/boojum-cuda/src/ntt.rs:179-291)."""
+    inputs = np.ascontiguousarray(inputs, dtype=np.uint64)
+    if out is None:
+        out = np.zeros(max(num_ntts * stride_out, 1), dtype=np.uint64)
+    rc = lib().gl_oracle_batch_ntt(_ptr(inputs), _ptr(out), log_n, num_ntts, stride_in, stride_out,
+                                   int(bitrev_inputs), int(inverse), log_ext, coset_idx, nthreads)
+    if rc != 0:
+        raise ValueError("invalid batch_ntt arguments")
+    return out
+
+
+def bit_reverse(src, log_rows, cols, src_stride=None, dst_stride=None, out=None):
+    n = 1 << log_rows
+    src_stride = n if src_stride is None else src_stride
+    dst_stride = n if dst_stride is None else dst_stride
+    src = np.ascontiguousarray(src, dtype=np.uint64)
+    if out is None:
+        out = np.zeros(cols * dst_stride, dtype=np.uint64)
+    lib().gl_oracle_bit_reverse(_ptr(src), src_stride, _ptr(out), dst_stride, log_rows, cols)
+    return out
+
+
+def lde(evals, log_n, num_cols, log_blowup, in_stride=None, out_col_stride=None,
+        out_coset_stride=None, out=None, nthreads=1):
+    n = 1 << log_n
+    in_stride = n if in_stride is None else in_stride
+    out_col_stride = (n << log_blowup) if out_col_stride is None else out_col_stride
+    out_coset_stride = n if out_coset_stride is None else out_coset_stride
+    evals = np.ascontiguousarray(evals, dtype=np.uint64)
+    if out is None:
+        size = (num_cols - 1) * out_col_stride + ((1 << log_blowup) - 1) * out_coset_stride + n
+        out = np.zeros(size, dtype=np.uint64)
+    rc = lib().gl_oracle_lde(_ptr(evals), in_stride, _ptr(out), out_col_stride, out_coset_stride,
+                             log_n, num_cols, log_blowup, nthreads)
+    if rc != 0:
+        raise ValueError("invalid lde arguments")
+    return out
+
+
+def fill_uniform(n, seed):
+    v = np.empty(n, dtype=np.uint64)
+    lib().gl_fill_uniform(_ptr(v), n, seed)
+    return v

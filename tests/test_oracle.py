@@ -1,0 +1,184 @@
+"""Pins the CPU oracle (oracle/gl_oracle.c) before anything is checked against it.
+
+Each check is chosen so that a plausible slip (wrong root, sign, twiddle index, order,
+dropped scaling) fails it:
+  * closed-form roots of unity and the generator's order
+  * field operations against Python big integers on edge and random values
+  * every batch mode against the independent O(n^2) big-integer DFT (tests/bigint_ref.py)
+  * the committed golden fixtures (tests/golden/ntt_golden.npz)
+  * transform invariants at sizes the brute force cannot reach
+  * LDE output equals direct polynomial evaluation at the coset points
+"""
+import os
+
+import numpy as np
+import pytest
+
+import bigint_ref as br
+import gl_oracle
+
+P = br.P
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ntt_golden.npz")
+
+
+def test_roots_closed_form():
+    w32 = pow(7, (P - 1) >> 32, P)
+    assert w32 == 0x185629DCDA58878C
+    assert gl_oracle.omega(32) == w32
+    # order exactly 2^32
+    assert pow(w32, 1 << 31, P) == P - 1
+    assert gl_oracle.omega(30) == 0x7E9BD009B86A0845
+    # omega_64 is the shift 2^39, omega_2 = -1
+    assert gl_oracle.omega(6) == pow(2, 39, P)
+    assert gl_oracle.omega(1) == P - 1
+    # 7 generates F_p^*: p - 1 = 2^32 * 3 * 5 * 17 * 257 * 65537
+    for q in (2, 3, 5, 17, 257, 65537):
+        assert pow(7, (P - 1) // q, P) != 1
+    assert gl_oracle.field_op("gl_inv", 7) == 0x249249246DB6DB6E
+    assert gl_oracle.field_op("gl_pow", 2, P - 1 - 20) == 0xFFFFEFFF00001001  # 2^-20
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_field_ops(seed):
+    rng = np.random.default_rng(seed)
+    edge = [0, 1, 2, P - 2, P - 1, 2**32 - 1, 2**32, 2**63, 2**64 - 2**32]
+    vals = edge + [int(v) % P for v in rng.integers(0, 2**64, 200, dtype=np.uint64)]
+    for a in vals:
+        for b in vals[:20]:
+            assert gl_oracle.field_op("gl_add", a, b) == (a + b) % P
+            assert gl_oracle.field_op("gl_sub", a, b) == (a - b) % P
+            assert gl_oracle.field_op("gl_mul", a, b) == (a * b) % P
+    for x in [P, P + 1, 2**64 - 1]:
+        assert gl_oracle.field_op("gl_reduce", x) == x % P
+        assert gl_oracle.field_op("gl_mul", x, 1) == x % P
+
+
+def test_twiddles_bitreversed_prefix():
+    big = gl_oracle.twiddles(10, False)
+    for log_n in range(1, 10):
+        tw = gl_oracle.twiddles(log_n, False)
+        assert np.array_equal(tw, big[: len(tw)])
+    w = br.omega(10)
+    for k in range(512):
+        assert int(big[k]) == pow(w, br.bitrev(k, 9), P)
+    inv = gl_oracle.twiddles(10, True)
+    for k in range(512):
+        assert int(inv[k]) * int(big[k]) % P == 1
+
+
+@pytest.mark.parametrize("bitrev_inputs,inverse", [(False, False), (True, False), (False, True), (True, True)])
+@pytest.mark.parametrize("log_ext,coset_idx", [(0, 0), (3, 3), (1, 0), (2, 1)])
+def test_batch_modes_vs_bruteforce(bitrev_inputs, inverse, log_ext, coset_idx):
+    for log_n in range(1, 7):
+        n = 1 << log_n
+        cols, stride = 3, 2 * n
+        rng = np.random.default_rng(log_n * 31 + coset_idx)
+        x = rng.integers(0, 2**64, cols * stride, dtype=np.uint64)
+        y = gl_oracle.batch_ntt(x, log_n, cols, stride, stride, bitrev_inputs, inverse, log_ext, coset_idx)
+        for c in range(cols):
+            ref = br.batch_mode([int(t) for t in x[c * stride: c * stride + n]], log_n, bitrev_inputs,
+                                inverse, log_ext, coset_idx)
+            assert [int(t) for t in y[c * stride: c * stride + n]] == ref
+
+
+def test_golden_fixtures():
+    g = np.load(GOLDEN)
+    for log_n in range(1, 10):
+        x = g[f"in_{log_n}"]
+        n = 1 << log_n
+        for b in (0, 1):
+            for i in (0, 1):
+                for log_ext, coset_idx in [(0, 0), (3, 3), (1, 0)]:
+                    y = gl_oracle.batch_ntt(x, log_n, 1, n, n, b, i, log_ext, coset_idx)
+                    assert np.array_equal(y, g[f"ntt_{log_n}_{b}{i}_{log_ext}_{coset_idx}"])
+    for log_n in range(1, 8):
+        ev = g[f"lde_in_{log_n}"]
+        for lb in (1, 2, 3):
+            assert np.array_equal(gl_oracle.lde(ev, log_n, 1, lb), g[f"lde_{log_n}_{lb}"])
+
+
+@pytest.mark.parametrize("log_n", [10, 13])
+def test_invariants(log_n):
+    n = 1 << log_n
+    rng = np.random.default_rng(log_n)
+    x = gl_oracle.fill_uniform(n, 77 + log_n)
+    # inverse(forward) == identity, both orderings
+    y = gl_oracle.batch_ntt(x, log_n, 1, n, n, False, False)
+    z = gl_oracle.batch_ntt(y, log_n, 1, n, n, True, True)
+    assert np.array_equal(z, x)
+    # coset forward then coset inverse is identity too
+    y = gl_oracle.batch_ntt(x, log_n, 1, n, n, False, False, 2, 3)
+    z = gl_oracle.batch_ntt(y, log_n, 1, n, n, True, True, 2, 3)
+    assert np.array_equal(z, x)
+    # delta_0 -> all ones; e_1 -> omega^k in bit-reversed order
+    d = np.zeros(n, dtype=np.uint64)
+    d[0] = 1
+    assert np.all(gl_oracle.batch_ntt(d, log_n, 1, n, n, False, False) == 1)
+    d[0], d[1] = 0, 1
+    y = gl_oracle.batch_ntt(d, log_n, 1, n, n, False, False)
+    w = br.omega(log_n)
+    for pos in rng.integers(0, n, 64):
+        assert int(y[pos]) == pow(w, br.bitrev(int(pos), log_n), P)
+    # linearity: NTT(a*x + y) == a*NTT(x) + NTT(y)
+    a = 0x123456789ABCDEF
+    x2 = gl_oracle.fill_uniform(n, 5)
+    lhs_in = np.array([(a * int(u) + int(v)) % P for u, v in zip(x, x2)], dtype=np.uint64)
+    lhs = gl_oracle.batch_ntt(lhs_in, log_n, 1, n, n, False, False, 1, 1)
+    fx = gl_oracle.batch_ntt(x, log_n, 1, n, n, False, False, 1, 1)
+    fy = gl_oracle.batch_ntt(x2, log_n, 1, n, n, False, False, 1, 1)
+    assert all(int(l) == (a * int(u) + int(v)) % P for l, u, v in zip(lhs[:256], fx[:256], fy[:256]))
+
+
+@pytest.mark.parametrize("log_n,log_blowup", [(8, 1), (9, 2), (6, 3)])
+def test_lde_equals_direct_evaluation(log_n, log_blowup):
+    n = 1 << log_n
+    ev = gl_oracle.fill_uniform(2 * n, 9)
+    out = gl_oracle.lde(ev, log_n, 2, log_blowup)
+    big = n << log_blowup
+    for c in range(2):
+        mono_br = gl_oracle.batch_ntt(ev[c * n:(c + 1) * n], log_n, 1, n, n, False, True)
+        coeffs = [int(t) for t in gl_oracle.bit_reverse(mono_br, log_n, 1)]
+        # monomials via n2b inverse + bit reverse: they must interpolate the evaluations
+        w = br.omega(log_n)
+        for j in (0, 1, n - 1):
+            assert br.horner(coeffs, pow(w, j, P)) == int(ev[c * n + j])
+        for q in list(range(8)) + [big - 1, big // 2 + 3]:
+            assert int(out[c * big + q]) == br.horner(coeffs, br.lde_point(log_n, log_blowup, q))
+
+
+def test_rejects_reference_asserts():
+    x = np.zeros(16, dtype=np.uint64)
+    with pytest.raises(ValueError):
+        gl_oracle.batch_ntt(x, 0, 1, 16, 16, False, False)
+    with pytest.raises(ValueError):
+        gl_oracle.batch_ntt(x, 4, 1, 8, 16, False, False)
+    with pytest.raises(ValueError):
+        gl_oracle.batch_ntt(x, 4, 1, 16, 16, False, False, 0, 1)
+    with pytest.raises(ValueError):
+        gl_oracle.batch_ntt(x, 4, 1, 16, 16, False, False, 2, 4)
+
+
+def test_bit_reverse():
+    log_rows, cols = 6, 3
+    n = 1 << log_rows
+    src = np.arange(cols * n, dtype=np.uint64)
+    dst = gl_oracle.bit_reverse(src, log_rows, cols)
+    for c in range(cols):
+        for i in range(n):
+            assert dst[c * n + br.bitrev(i, log_rows)] == src[c * n + i]
+
+
+def test_fill_uniform_matches_splitmix():
+    def mix(z):
+        M = 2**64 - 1
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return z ^ (z >> 31)
+
+    g = 0x9E3779B97F4A7C15
+    v = gl_oracle.fill_uniform(100, 42)
+    for i in range(100):
+        x = mix((42 + (i + 1) * g) & (2**64 - 1))
+        while x >= P:
+            x = mix(x ^ g)
+        assert int(v[i]) == x

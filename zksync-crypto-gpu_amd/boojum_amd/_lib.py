@@ -1,0 +1,88 @@
+"""Loads libboojum_amd.so and declares its C-ABI (include/boojum_amd.h)."""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(os.path.dirname(_HERE), "libboojum_amd.so")
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_vp = ctypes.c_void_p
+_u32, _u64, _i32, _sz = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_size_t
+
+# name -> argtypes (restype int unless listed in _RESTYPES)
+SIGNATURES = {
+    "ba_get_device_count": [ctypes.POINTER(_i32)],
+    "ba_set_device": [_i32],
+    "ba_get_device": [ctypes.POINTER(_i32)],
+    "ba_device_synchronize": [],
+    "ba_mem_get_info": [ctypes.POINTER(_sz), ctypes.POINTER(_sz)],
+    "ba_device_get_attribute": [ctypes.POINTER(_i32), _i32, _i32],
+    "ba_malloc": [ctypes.POINTER(_vp), _sz],
+    "ba_free": [_vp],
+    "ba_malloc_host": [ctypes.POINTER(_vp), _sz],
+    "ba_free_host": [_vp],
+    "ba_memcpy": [_vp, _vp, _sz, _i32],
+    "ba_memcpy_async": [_vp, _vp, _sz, _i32, _vp],
+    "ba_memset_async": [_vp, _i32, _sz, _vp],
+    "ba_stream_create": [ctypes.POINTER(_vp)],
+    "ba_stream_destroy": [_vp],
+    "ba_stream_synchronize": [_vp],
+    "ba_stream_wait_event": [_vp, _vp],
+    "ba_event_create": [ctypes.POINTER(_vp)],
+    "ba_event_destroy": [_vp],
+    "ba_event_record": [_vp, _vp],
+    "ba_event_synchronize": [_vp],
+    "ba_event_query": [_vp],
+    "ba_event_elapsed_ms": [ctypes.POINTER(ctypes.c_float), _vp, _vp],
+    "ba_get_last_error": [],
+    "ba_error_name": [_i32],
+    "ba_stream_begin_capture": [_vp],
+    "ba_stream_end_capture": [_vp, ctypes.POINTER(_vp)],
+    "ba_graph_launch": [_vp, _vp],
+    "ba_graph_destroy": [_vp],
+    "ba_context_create": [_u32, _u32, ctypes.POINTER(_vp)],
+    "ba_context_destroy": [_vp],
+    "ba_batch_ntt_internal": [_vp, _vp, _u32, _u32, _u32, _u32, _i32, _i32, _u32, _u32, _vp],
+    "ba_batch_ntt_out_of_place": [_vp, _sz, _vp, _sz, _u32, _u32, _u32, _u32, _u32, _u32, _i32,
+                                  _i32, _u32, _u32, _vp],
+    "ba_batch_ntt_in_place": [_vp, _sz, _u32, _u32, _u32, _u32, _i32, _i32, _u32, _u32, _vp],
+    "ba_batch_ntt_ctx": [_vp, _vp, _vp, _u32, _u32, _u64, _u64, _i32, _i32, _u32, _u32, _vp],
+    "ba_bit_reverse": [_vp, _u64, _vp, _u64, _u32, _u32, _vp],
+    "ba_bit_reverse_in_place": [_vp, _u64, _u32, _u32, _vp],
+    "ba_fill_uniform": [_vp, _u64, _u64, _u32, _u64, _vp],
+    "ba_field_op": [_i32, _vp, _vp, _vp, _sz, _vp],
+}
+_RESTYPES = {"ba_error_name": ctypes.c_char_p}
+
+_lib = None
+
+
+class BoojumAmdError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        name = "?"
+        try:
+            name = lib().ba_error_name(code).decode()
+        except Exception:  # pragma: no cover - only when the library itself is broken
+            pass
+        super().__init__(f"{what}: status {code} ({name})")
+
+
+def lib():
+    """The loaded library; raises if it was not built (there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(library_path):
+            raise RuntimeError(f"libboojum_amd.so not built at {library_path}; run __graft_entry__.build()")
+        L = ctypes.CDLL(library_path)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def check(rc, what="boojum_amd call"):
+    if rc != 0:
+        raise BoojumAmdError(rc, what)

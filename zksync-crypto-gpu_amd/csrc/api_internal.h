@@ -1,0 +1,43 @@
+// api_internal.h -- definitions shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/boojum_amd.h"
+#include "gl_tables.h"
+
+struct ba_context {
+  int device;
+  uint32_t w_coarse_log;
+  uint32_t g_coarse_log;
+  void *mem;
+  size_t bytes;
+  gl::Tables tab;
+};
+
+// The most recently created live context of `device`, or null.
+ba_context *ba_current_context(int device);
+
+// Context of the current device for the context-free entry points.
+inline ba_context *ba_context_for_current_device() {
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess) return nullptr;
+  return ba_current_context(device);
+}
+
+// Launch-time plan of one NTT: the index bits each pass transforms, in execution order.
+struct NttPlan {
+  int num_passes;
+  uint32_t lo_bit[8];
+  uint32_t bits[8];
+  bool contig[8];
+};
+
+// Pass decomposition for a 2^log_n transform (n2b order: strided passes from the top bits
+// down, then the contiguous pass; b2n runs the same passes in reverse).
+NttPlan ba_plan_ntt(uint32_t log_n, bool n2b);
+
+// Enqueues one batch NTT (arguments already validated).
+int ba_launch_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint32_t log_n,
+                  uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs,
+                  bool inverse, uint32_t log_ext, uint32_t coset_idx, hipStream_t stream);

@@ -1,0 +1,70 @@
+// helpers.hip -- seeded device input generation and an element-wise field-op kernel that
+// exposes the NTT kernels' arithmetic to the parity tests.
+#include <algorithm>
+
+#include "api_internal.h"
+
+namespace {
+
+using gl::u32;
+using gl::u64;
+
+__device__ __forceinline__ u64 splitmix(u64 z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+constexpr u64 kGamma = 0x9E3779B97F4A7C15ull;
+constexpr u64 kColumnStep = 0xD1B54A32D192ED03ull;
+
+__global__ __launch_bounds__(256) void fill_uniform_kernel(u64 *dst, u64 stride, u64 rows, u32 cols,
+                                                           u64 seed) {
+  const u64 total = rows * cols;
+  for (u64 k = blockIdx.x * 256ull + threadIdx.x; k < total; k += (u64)gridDim.x * 256) {
+    const u64 col = k / rows, i = k - col * rows;
+    const u64 s = seed + col * kColumnStep;
+    u64 x = splitmix(s + (i + 1) * kGamma);
+    while (x >= gl::P) x = splitmix(x ^ kGamma);
+    dst[col * stride + i] = x;
+  }
+}
+
+__global__ __launch_bounds__(256) void field_op_kernel(int op, const u64 *a, const u64 *b, u64 *out,
+                                                       size_t n) {
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    const u64 x = gl::canon(a[i]);
+    const u64 y = b ? gl::canon(b[i]) : 0;
+    u64 r;
+    switch (op) {
+      case 0: r = gl::add(x, y); break;
+      case 1: r = gl::sub(x, y); break;
+      case 2: r = gl::mul(a[i], b[i]); break;  // mul accepts unreduced operands
+      case 3: r = x; break;
+      default: r = gl::inv(x); break;
+    }
+    out[i] = r;
+  }
+}
+
+}  // namespace
+
+extern "C" int ba_fill_uniform(uint64_t *dst, uint64_t stride, uint64_t rows, uint32_t cols,
+                               uint64_t seed, ba_stream_t stream) {
+  if (!dst || (cols > 1 && stride < rows)) return BA_ERROR_INVALID_VALUE;
+  if (rows == 0 || cols == 0) return BA_SUCCESS;
+  const u64 total = rows * cols;
+  const u32 grid = (u32)std::min<u64>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid), dim3(256), 0, (hipStream_t)stream, dst, stride,
+                     rows, cols, seed);
+  return (int)hipGetLastError();
+}
+
+extern "C" int ba_field_op(int op, const uint64_t *a, const uint64_t *b, uint64_t *out, size_t n,
+                           ba_stream_t stream) {
+  if (op < 0 || op > 4 || !a || !out || ((op <= 2) && !b)) return BA_ERROR_INVALID_VALUE;
+  if (n == 0) return BA_SUCCESS;
+  const u32 grid = (u32)std::min<size_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(field_op_kernel, dim3(grid), dim3(256), 0, (hipStream_t)stream, op, a, b, out, n);
+  return (int)hipGetLastError();
+}
