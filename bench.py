@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark: batched Goldilocks coset LDE on one MI355X.
+
+Workload (the headline configuration): 64 columns of 2^20 uniform field elements, interpolated
+and evaluated on 2 cosets of the blowup-2 domain with shift 7 (inverse NTT, coset scale, forward
+NTT, bit-reversed output: 64 x 2^21 output elements per step), the whole LDE replayed as one
+captured hipGraph.  Inputs are generated on the device and resident in HBM before timing.
+
+Printed (one JSON line): Melem/s of output, the batched-vs-per-column-call speedup, the
+roofline of the dominant kernel (timed per launch with HIP events through the library's launch
+tracing), and the CPU baseline (the oracle's restatement of boojum's CPU NTT on a bounded sample
+of columns, on this host's cores).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "zksync-crypto-gpu_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of each kernel on this workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# runs scaled by a copy kernel (profiles/lde_64x2p20_blowup2_passes.md)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lde_64x2p20_blowup2_traffic.json")
+METRIC = "LDE Melem/s + % of HBM roofline, batched vs per-call; serve p50/p99 ms"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log-n", type=int, default=20)
+    ap.add_argument("--cols", type=int, default=64)
+    ap.add_argument("--log-blowup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-per-call", action="store_true", help="skip the per-column-call leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a 2^27-element copy kernel (canonicalise, 8 B read + 8 B written "
+                         "per element) that scales the FETCH_SIZE/WRITE_SIZE counters")
+    return ap.parse_args()
+
+
+def cpu_baseline(log_n, log_blowup):
+    """The oracle (plain-C restatement of boojum's CPU NTT) on a bounded sample of columns."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import gl_oracle
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    cols = threads
+    n = 1 << log_n
+    ev = gl_oracle.fill_uniform(cols * n, 7)
+    t0 = time.perf_counter()
+    gl_oracle.lde(ev, log_n, cols, log_blowup, nthreads=threads)
+    dt = time.perf_counter() - t0
+    out_elems = cols * (n << log_blowup)
+    return {
+        "value": round(out_elems / dt / 1e6, 3),
+        "unit": "Melem/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{cols} columns of the same 2^{log_n} blowup-{1 << log_blowup} LDE, one per thread, "
+                  f"{dt:.2f} s wall",
+    }
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: every box in this build has one MI355X and the LDE shards "
+              "by columns with no communication; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import boojum_amd as ba
+    from boojum_amd.profile import LaunchTrace
+
+    assert ba.device_count() >= 1, "no GPU visible"
+    ba.set_device(0)
+    ctx = ba.Context.create(12, 12)
+    log_n, cols, lb = args.log_n, args.cols, args.log_blowup
+    n, B = 1 << log_n, 1 << lb
+    d_ev = ba.DeviceAllocation(n * cols)
+    d_mono = ba.DeviceAllocation(n * cols)
+    d_out = ba.DeviceAllocation(B * n * cols)
+    ba.runtime.fill_uniform(d_ev, n, cols, 1)
+    stream = ba.CudaStream()
+    if args.calibrate_copy:
+        cn = 1 << 27
+        c_src, c_dst = ba.DeviceAllocation(cn), ba.DeviceAllocation(cn)
+        ba.runtime.fill_uniform(c_src, cn, 1, 3)
+        for _ in range(3):
+            ba.runtime.field_op(3, c_src, None, c_dst, cn)
+        ba.synchronize()
+        c_src.free()
+        c_dst.free()
+
+    def step_eager(s=stream):
+        ba.lde.lde_batch(ctx, d_ev, log_n, cols, lb, d_out, monomials=d_mono, stream=s)
+
+    # capture the whole LDE (iNTT, coset scale, forward NTTs) as one graph
+    import ctypes
+    lib = ba.lib()
+    ba.check(lib.ba_stream_begin_capture(stream.handle), "begin_capture")
+    step_eager()
+    graph = ctypes.c_void_p(None)
+    ba.check(lib.ba_stream_end_capture(stream.handle, ctypes.byref(graph)), "end_capture")
+
+    def step():
+        ba.check(lib.ba_graph_launch(graph, stream.handle), "graph_launch")
+
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+    ba.synchronize()
+    e0, e1 = ba.CudaEvent(), ba.CudaEvent()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    e1.synchronize()
+    ba.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = e0.elapsed_ms(e1) / args.steps
+    out_elems = cols * B * n
+    value = out_elems / (ms_per_step / 1e3) / 1e6
+    lde_alg_bytes = 8 * n * (1 + B) * cols
+
+    # dominant kernel: one traced eager step, per-launch HIP events on the launch stream
+    per_kernel = {}
+    trace_steps = max(1, min(5, args.steps))
+    with LaunchTrace(64 * trace_steps) as tr:
+        for _ in range(trace_steps):
+            step_eager()
+    stream.synchronize()
+    for name, ms in tr.durations_ms():
+        tot, cnt = per_kernel.get(name, (0.0, 0))
+        per_kernel[name] = (tot + ms, cnt + 1)
+    tr.destroy()
+    dom_name, (dom_tot, dom_cnt) = max(per_kernel.items(), key=lambda kv: kv[1][0])
+    dom_avg_ms = dom_tot / dom_cnt
+    # each pass launch reads and writes every element of the batch once: 16 B per element
+    dom_bytes = 16 * n * cols
+    achieved = dom_bytes / (dom_avg_ms / 1e3) / 1e9
+    traffic = None
+    if (log_n, cols, lb) == (20, 64, 1) and os.path.exists(TRAFFIC_FILE):
+        t = json.load(open(TRAFFIC_FILE))["traffic_bytes_per_launch"].get(dom_name)
+        traffic = int(t) if t is not None else None
+
+    per_call = None
+    if not args.no_per_call:
+        # one ba_lde_batch call per column (6 launches each), eager, as a caller without batching
+        pc_steps = max(1, min(3, args.steps))
+        for _ in range(1):
+            for k in range(cols):
+                ba.lde.lde_batch(ctx, d_ev.sub(k * n, n), log_n, 1, lb, d_out.sub(k * B * n, B * n),
+                                 monomials=d_mono.sub(k * n, n), stream=stream)
+        stream.synchronize()
+        p0, p1 = ba.CudaEvent(), ba.CudaEvent()
+        p0.record(stream)
+        for _ in range(pc_steps):
+            for k in range(cols):
+                ba.lde.lde_batch(ctx, d_ev.sub(k * n, n), log_n, 1, lb, d_out.sub(k * B * n, B * n),
+                                 monomials=d_mono.sub(k * n, n), stream=stream)
+        p1.record(stream)
+        p1.synchronize()
+        pc_ms = p0.elapsed_ms(p1) / pc_steps
+        per_call = {"ms_per_step": round(pc_ms, 4), "melem_s": round(out_elems / (pc_ms / 1e3) / 1e6, 1),
+                    "batched_speedup": round(pc_ms / ms_per_step, 3)}
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Melem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 (Goldilocks field, exact)",
+        "data": "synthetic: uniform field elements from a seeded SplitMix64 stream, generated in HBM",
+        "config": {"workload": f"batched coset LDE, {cols} columns x 2^{log_n}, blowup {B}, shift 7, "
+                               f"bit-reversed output, one hipGraph per step",
+                   "columns": cols, "log_n": log_n, "blowup": B, "parallelism": "single GPU"},
+        "lde_roofline": {"alg_bytes_per_step": lde_alg_bytes,
+                         "achieved_GBs": round(lde_alg_bytes / (ms_per_step / 1e3) / 1e9, 1),
+                         "frac": round(lde_alg_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "avg_launch_ms": round(dom_avg_ms, 4),
+                     "alg_bytes_per_launch": dom_bytes},
+        "per_kernel_ms": {k: round(v[0] / v[1], 4) for k, v in per_kernel.items()},
+        "per_call": per_call,
+        "wall_s": round(wall, 3),
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(log_n, lb)
+    ba.check(lib.ba_graph_destroy(graph), "graph_destroy")
+    for t in (d_ev, d_mono, d_out):
+        t.free()
+    stream.destroy()
+    ctx.destroy()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

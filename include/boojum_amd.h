@@ -134,6 +134,32 @@ int ba_bit_reverse(const uint64_t *src, uint64_t src_stride, uint64_t *dst, uint
 int ba_bit_reverse_in_place(uint64_t *values, uint64_t stride, uint32_t log_rows, uint32_t cols,
                             ba_stream_t stream);
 
+/* ---------------------------------------------------------------- coset LDE
+ * shivini's trace-commitment LDE over num_cols columns (crates/shivini/src/data_structures/
+ * storage.rs:154-260 with primitives/ntt.rs:10-42): evaluations (natural order) are
+ * interpolated to monomials, then for each coset c < 2^log_blowup the coset evaluations
+ * f(7 * omega_{n 2^log_blowup}^bitrev(c) * omega_n^bitrev(i)) are written bit-reversed to
+ * out + k*out_col_stride + c*out_coset_stride.  With out_col_stride = n << log_blowup and
+ * out_coset_stride = n a column holds its whole LDE domain in bit-reversed order; with
+ * out_col_stride = n and out_coset_stride = num_cols * n the cosets are separate matrices, as
+ * shivini stores them.  The monomials, in bit-reversed order, go to `monomials` (stride n)
+ * when it is non-null, else they replace the evaluations in place (as into_monomials consumes
+ * its storage, storage.rs:155).  Preconditions: 1 <= log_n, 1 <= log_blowup,
+ * log_n + log_blowup <= 30, strides >= n, out distinct from evals / monomials. */
+int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_stride, uint64_t *monomials,
+                 uint64_t *out, uint64_t out_col_stride, uint64_t out_coset_stride, uint32_t log_n,
+                 uint32_t num_cols, uint32_t log_blowup, ba_stream_t stream);
+
+/* ---------------------------------------------------------------- launch tracing
+ * No reference counterpart beyond its nvtx ranges (crates/shivini/Cargo.toml:31): after
+ * ba_profile_begin(events, capacity) on a host thread, each kernel the library launches from
+ * that thread is bracketed by events[2i] / events[2i+1] (caller-created) for its first
+ * `capacity` launches; ba_profile_end reports how many were recorded and ba_profile_launch_name
+ * the kernel name (as rocprofv3 prints it) of launch i. */
+int ba_profile_begin(ba_event_t *events, uint32_t capacity);
+int ba_profile_end(uint32_t *launches);
+const char *ba_profile_launch_name(uint32_t index);
+
 /* ---------------------------------------------------------------- helpers
  * Seeded uniform field elements on the device: element i of column k is the SplitMix64
  * finaliser of s_k + (i+1) * 0x9E3779B97F4A7C15 with s_k = seed + k * 0xD1B54A32D192ED03,

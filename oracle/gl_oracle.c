@@ -21,14 +21,36 @@ typedef unsigned __int128 u128;
 
 uint64_t gl_reduce(uint64_t x) { return x >= GL_P ? x - GL_P : x; }
 
+/* 2^64 = 2^32 - 1 and 2^96 = -1 (mod p) */
+#define GL_EPS 0xFFFFFFFFull
+
 uint64_t gl_add(uint64_t a, uint64_t b) {
-  u128 s = (u128)a + b;
-  return (uint64_t)(s % GL_P);
+  a = gl_reduce(a);
+  b = gl_reduce(b);
+  uint64_t s = a + b;
+  if (s < a) return s + GL_EPS; /* a + b - 2^64 + (2^64 - p) < p */
+  return gl_reduce(s);
 }
 
-uint64_t gl_sub(uint64_t a, uint64_t b) { return a >= b ? a - b : a + (GL_P - b); }
+uint64_t gl_sub(uint64_t a, uint64_t b) {
+  a = gl_reduce(a);
+  b = gl_reduce(b);
+  return a >= b ? a - b : a + (GL_P - b);
+}
 
-uint64_t gl_mul(uint64_t a, uint64_t b) { return (uint64_t)(((u128)a * b) % GL_P); }
+/* lo + hi 2^64 = lo - hh + hl (2^32 - 1) with hi = hh 2^32 + hl */
+static inline uint64_t reduce128(u128 x) {
+  uint64_t lo = (uint64_t)x, hi = (uint64_t)(x >> 64);
+  uint64_t hh = hi >> 32, hl = hi & GL_EPS;
+  uint64_t t0 = lo - hh;
+  if (lo < hh) t0 -= GL_EPS;
+  uint64_t t1 = hl * GL_EPS;
+  uint64_t r = t0 + t1;
+  if (r < t1) r += GL_EPS;
+  return gl_reduce(r);
+}
+
+uint64_t gl_mul(uint64_t a, uint64_t b) { return reduce128((u128)a * b); }
 
 uint64_t gl_pow(uint64_t a, uint64_t e) {
   uint64_t r = 1, x = gl_reduce(a);

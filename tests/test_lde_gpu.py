@@ -1,0 +1,137 @@
+"""GPU parity of the batched coset LDE (ba_lde_batch) against the oracle's restatement of
+shivini's composition (crates/shivini/src/data_structures/storage.rs:154-260)."""
+import numpy as np
+import pytest
+
+import gl_oracle
+from gpu_util import P, bitrev_perm, log, raw_u64
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ba():
+    import boojum_amd
+    assert boojum_amd.device_count() >= 1, "no GPU visible"
+    ctx = boojum_amd.Context.create(12, 12)
+    boojum_amd._test_ctx = ctx
+    yield boojum_amd
+    ctx.destroy()
+
+
+def run_lde(ba, evals_host, log_n, cols, log_blowup, layout="column", keep_evals=False, in_stride=None):
+    n = 1 << log_n
+    B = 1 << log_blowup
+    in_stride = n if in_stride is None else in_stride
+    d_ev = ba.DeviceAllocation.from_host(evals_host)
+    if layout == "column":
+        col_stride, coset_stride = B * n, n
+    else:  # coset-major, as shivini stores each coset as its own matrix
+        col_stride, coset_stride = n, cols * n
+    d_out = ba.DeviceAllocation(B * n * cols)
+    d_mono = ba.DeviceAllocation(n * cols) if keep_evals else None
+    try:
+        ba.lde.lde_batch(ba._test_ctx, d_ev, log_n, cols, log_blowup, d_out, in_stride=in_stride,
+                         monomials=d_mono, out_col_stride=col_stride, out_coset_stride=coset_stride)
+        ba.synchronize()
+        out = d_out.to_host()
+        ev_after = d_ev.to_host()
+        mono = d_mono.to_host() if d_mono is not None else None
+    finally:
+        d_ev.free()
+        d_out.free()
+        if d_mono is not None:
+            d_mono.free()
+    # normalise to column-contiguous layout
+    if layout != "column":
+        out = np.concatenate([np.concatenate([out[c * cols * n + k * n: c * cols * n + (k + 1) * n]
+                                              for c in range(B)]) for k in range(cols)])
+    return out, ev_after, mono
+
+
+@pytest.mark.parametrize("log_blowup", [1, 2, 3])
+def test_lde_small_sizes(ba, log_blowup):
+    for log_n in list(range(1, 15)) + [16]:
+        cols = 3
+        n = 1 << log_n
+        log(f"lde parity log_n={log_n} blowup=2^{log_blowup}")
+        ev = raw_u64(np.random.default_rng(log_n * 7 + log_blowup), cols * n)
+        want = gl_oracle.lde(ev, log_n, cols, log_blowup, nthreads=4)
+        got, ev_after, _ = run_lde(ba, ev, log_n, cols, log_blowup)
+        assert np.array_equal(got, want), (log_n, log_blowup)
+        # the evaluations were replaced by the monomials in bit-reversed order
+        mono_br = gl_oracle.batch_ntt(ev, log_n, cols, n, n, False, True)
+        assert np.array_equal(ev_after, mono_br)
+
+
+@pytest.mark.parametrize("layout", ["column", "coset"])
+def test_lde_layouts_and_scratch(ba, layout):
+    for log_n in (5, 12, 13, 17):
+        cols, lb = 5, 2
+        n = 1 << log_n
+        log(f"lde layout={layout} log_n={log_n}")
+        ev = raw_u64(np.random.default_rng(log_n), cols * (n + 3))
+        want = gl_oracle.lde(ev, log_n, cols, lb, in_stride=n + 3, nthreads=4)
+        got, ev_after, mono = run_lde(ba, ev, log_n, cols, lb, layout=layout, keep_evals=True, in_stride=n + 3)
+        assert np.array_equal(got, want)
+        assert np.array_equal(ev_after, ev)  # evaluations kept when a monomial buffer is given
+        assert np.array_equal(mono, gl_oracle.batch_ntt(ev, log_n, cols, n + 3, n, False, True))
+
+
+def test_lde_golden(ba, golden):
+    for log_n in range(1, 8):
+        ev = golden[f"lde_in_{log_n}"]
+        for lb in (1, 2, 3):
+            got, _, _ = run_lde(ba, ev, log_n, 1, lb)
+            assert np.array_equal(got, golden[f"lde_{log_n}_{lb}"])
+
+
+def test_lde_headline_size_samples(ba):
+    """64 columns of 2^20 at blowup 2 (the benchmark's shape): two columns checked against the
+    oracle, every column through the interpolation round trip."""
+    log_n, cols, lb = 20, 64, 1
+    n, B = 1 << log_n, 2
+    d_ev = ba.DeviceAllocation(n * cols)
+    d_keep = ba.DeviceAllocation(n * cols)
+    d_out = ba.DeviceAllocation(B * n * cols)
+    d_tmp = ba.DeviceAllocation(n * cols)
+    try:
+        ba.runtime.fill_uniform(d_ev, n, cols, 2024)
+        d_keep.copy_from_device(d_ev)
+        ba.lde.lde_batch(ba._test_ctx, d_ev, log_n, cols, lb, d_out)
+        ba.synchronize()
+        ev = d_keep.to_host()
+        out = d_out.to_host()
+        for k in (0, 37):
+            want = gl_oracle.lde(ev[k * n:(k + 1) * n], log_n, 1, lb, nthreads=8)
+            assert np.array_equal(out[k * B * n:(k + 1) * B * n], want), k
+        # every coset interpolates back to the evaluations:
+        # b2n inverse with the coset factors -> natural monomials -> n2b forward -> bitrev evals
+        perm = bitrev_perm(log_n)
+        for c in range(B):
+            coset_idx = c  # bitrev over one bit is the identity
+            for k in range(cols):
+                d_tmp.sub(k * n, n).copy_from_device(d_out.sub(k * B * n + c * n, n))
+            ba.ntt.batch_ntt_internal(d_tmp.ptr, d_tmp.ptr, log_n, cols, n, n, True, True, lb, coset_idx)
+            ba.ntt.batch_ntt_internal(d_tmp.ptr, d_tmp.ptr, log_n, cols, n, n, False, False, 0, 0)
+            ba.synchronize()
+            back = d_tmp.to_host()
+            for k in range(cols):
+                assert np.array_equal(back[k * n:(k + 1) * n][perm], ev[k * n:(k + 1) * n]), (c, k)
+    finally:
+        for t in (d_ev, d_keep, d_out, d_tmp):
+            t.free()
+
+
+def test_lde_rejects_bad_arguments(ba):
+    d = ba.DeviceAllocation(1 << 10)
+    o = ba.DeviceAllocation(1 << 12)
+    try:
+        for log_n, lb in [(0, 1), (5, 0), (28, 3)]:
+            with pytest.raises(ba.BoojumAmdError):
+                ba.lde.lde_batch(ba._test_ctx, d, log_n, 1, lb, o)
+        with pytest.raises(ba.BoojumAmdError):
+            ba.lde.lde_batch(ba._test_ctx, d, 5, 1, 1, d)  # output aliases evaluations
+    finally:
+        d.free()
+        o.free()

@@ -49,10 +49,14 @@ SIGNATURES = {
     "ba_batch_ntt_ctx": [_vp, _vp, _vp, _u32, _u32, _u64, _u64, _i32, _i32, _u32, _u32, _vp],
     "ba_bit_reverse": [_vp, _u64, _vp, _u64, _u32, _u32, _vp],
     "ba_bit_reverse_in_place": [_vp, _u64, _u32, _u32, _vp],
+    "ba_lde_batch": [_vp, _vp, _u64, _vp, _vp, _u64, _u64, _u32, _u32, _u32, _vp],
+    "ba_profile_begin": [ctypes.POINTER(_vp), _u32],
+    "ba_profile_end": [ctypes.POINTER(_u32)],
+    "ba_profile_launch_name": [_u32],
     "ba_fill_uniform": [_vp, _u64, _u64, _u32, _u64, _vp],
     "ba_field_op": [_i32, _vp, _vp, _vp, _sz, _vp],
 }
-_RESTYPES = {"ba_error_name": ctypes.c_char_p}
+_RESTYPES = {"ba_error_name": ctypes.c_char_p, "ba_profile_launch_name": ctypes.c_char_p}
 
 _lib = None
 

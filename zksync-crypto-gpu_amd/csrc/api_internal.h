@@ -37,7 +37,15 @@ struct NttPlan {
 // down, then the contiguous pass; b2n runs the same passes in reverse).
 NttPlan ba_plan_ntt(uint32_t log_n, bool n2b);
 
-// Enqueues one batch NTT (arguments already validated).
+// Enqueues one batch NTT (arguments already validated).  in_bitrev_gather (natural -> bitrev
+// forward only): the input holds each column in bit-reversed order and the first pass gathers
+// it, so a bit-reversed monomial vector feeds a coset NTT without a separate permutation.
 int ba_launch_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint32_t log_n,
                   uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs,
-                  bool inverse, uint32_t log_ext, uint32_t coset_idx, hipStream_t stream);
+                  bool inverse, uint32_t log_ext, uint32_t coset_idx, hipStream_t stream,
+                  bool in_bitrev_gather = false);
+
+// Launch tracing: when a host thread has called ba_profile_begin, every kernel the library
+// launches from that thread is bracketed by two of the caller's events.
+void ba_trace_before(hipStream_t stream, const char *kernel);
+void ba_trace_after(hipStream_t stream);

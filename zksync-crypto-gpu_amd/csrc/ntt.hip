@@ -76,11 +76,27 @@ PassKernel pick_tiny(uint32_t log_n, bool n2b, bool inverse) {
 
 constexpr uint64_t kMaxGrid = 0x7FFFFFFFull;
 
+// names as rocprofv3 reports the kernels
+const char *kernel_name(bool contig, bool n2b, bool inverse) {
+  static const char *names[8] = {
+      "void gl::ntt::ntt_pass_strided<false, false>(gl::ntt::PassArgs)",
+      "void gl::ntt::ntt_pass_strided<false, true>(gl::ntt::PassArgs)",
+      "void gl::ntt::ntt_pass_strided<true, false>(gl::ntt::PassArgs)",
+      "void gl::ntt::ntt_pass_strided<true, true>(gl::ntt::PassArgs)",
+      "void gl::ntt::ntt_pass_contig<false, false>(gl::ntt::PassArgs)",
+      "void gl::ntt::ntt_pass_contig<false, true>(gl::ntt::PassArgs)",
+      "void gl::ntt::ntt_pass_contig<true, false>(gl::ntt::PassArgs)",
+      "void gl::ntt::ntt_pass_contig<true, true>(gl::ntt::PassArgs)",
+  };
+  return names[(contig ? 4 : 0) + (n2b ? 2 : 0) + (inverse ? 1 : 0)];
+}
+
 }  // namespace
 
 int ba_launch_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint32_t log_n,
                   uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs,
-                  bool inverse, uint32_t log_ext, uint32_t coset_idx, hipStream_t stream) {
+                  bool inverse, uint32_t log_ext, uint32_t coset_idx, hipStream_t stream,
+                  bool in_bitrev_gather) {
   if (num_cols == 0) return BA_SUCCESS;
   const bool n2b = !bitrev_inputs;
   PassArgs base{};
@@ -100,9 +116,12 @@ int ba_launch_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint
     a.bits = log_n;
     a.pre_scale = pre;
     a.post_scale = inverse;
+    a.in_bitrev = in_bitrev_gather;
     const uint32_t grid = (num_cols + gl::ntt::THREADS - 1) / gl::ntt::THREADS;
+    ba_trace_before(stream, "ntt_tiny");
     hipLaunchKernelGGL(pick_tiny(log_n, n2b, inverse), dim3(grid), dim3(gl::ntt::THREADS), 0,
                        stream, a);
+    ba_trace_after(stream);
     return (int)hipGetLastError();
   }
 
@@ -129,9 +148,12 @@ int ba_launch_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint
       a.bits = S;
       a.pre_scale = pre && p == 0;
       a.post_scale = inverse && p == plan.num_passes - 1;
+      a.in_bitrev = in_bitrev_gather && p == 0;
       const uint64_t grid = per_col_tiles ? per_col_tiles * nc : (nc + cols_per_tile - 1) / cols_per_tile;
+      ba_trace_before(stream, kernel_name(contig, n2b, inverse));
       hipLaunchKernelGGL(pick_kernel(contig, n2b, inverse), dim3((uint32_t)grid),
                          dim3(gl::ntt::THREADS), 0, stream, a);
+      ba_trace_after(stream);
       const hipError_t err = hipGetLastError();
       if (err != hipSuccess) return (int)err;
     }

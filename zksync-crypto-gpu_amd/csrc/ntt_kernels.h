@@ -157,6 +157,7 @@ struct PassArgs {
   u32 post_scale; // inverse: outputs *= n^-1 (s^-j when coset)
   u32 coset;      // log_extension_degree > 0
   u32 coset_mul;  // coset_index << (30 - L - log_ext): s^j = 7^j omega_{2^30}^(j * coset_mul)
+  u32 in_bitrev;  // first pass only: natural element j is read from position bitrev_L(j)
   Tables tab;
 };
 
@@ -275,13 +276,37 @@ __global__ __launch_bounds__(THREADS) void ntt_pass_strided(PassArgs a) {
   u64 v[VALS];
   u32 K, wb, ab;
   round_shape(N2B, S, 0, K, wb, ab);
-  static_for<VALS>([&](auto m_) {
-    constexpr int m = decltype(m_)::value;
-    const u32 idx = base + (window_a(q, wb, m) << P0);
-    u64 x = canon(in[idx]);
-    if (a.pre_scale) x = mul(x, scale_factor(a, N2B ? idx : bitrev32(idx, L), false));
-    v[m] = x;
-  });
+  if (a.in_bitrev) {
+    // Top pass (P0 + S == L) over an input stored bit-reversed: the natural rows a of the
+    // tile's column l sit, permuted, in the contiguous chunk [bitrev(l) 2^S, +2^S).  Stage
+    // them through LDS with coalesced chunk reads.
+    const u32 l0 = lblk << log_c;
+    static_for<VALS>([&](auto i_) {
+      constexpr int i = decltype(i_)::value;
+      const u32 e = t + THREADS * i;
+      const u32 cl = e >> S, off = e & ((1u << S) - 1);
+      const u32 l = l0 + cl;
+      const u32 arow = bitrev32(off, S);
+      u64 x = canon(in[(bitrev32(l, P0) << S) + off]);
+      const u32 j = (arow << P0) + l;
+      if (a.pre_scale) x = mul(x, scale_factor(a, N2B ? j : bitrev32(j, L), false));
+      lds[(arow << log_c) + cl] = x;
+    });
+    __syncthreads();
+    static_for<VALS>([&](auto m_) {
+      constexpr int m = decltype(m_)::value;
+      v[m] = lds[(window_a(q, wb, m) << log_c) + c];
+    });
+    __syncthreads();
+  } else {
+    static_for<VALS>([&](auto m_) {
+      constexpr int m = decltype(m_)::value;
+      const u32 idx = base + (window_a(q, wb, m) << P0);
+      u64 x = canon(in[idx]);
+      if (a.pre_scale) x = mul(x, scale_factor(a, N2B ? idx : bitrev32(idx, L), false));
+      v[m] = x;
+    });
+  }
   for (u32 r = 0;; r++) {
     run_round<N2B, INV>(v, a, K, wb, ab, base + (window_a(q, wb, 0) << P0));
     if (r + 1 == rounds) break;
@@ -326,7 +351,11 @@ __global__ __launch_bounds__(THREADS) void ntt_pass_contig(PassArgs a) {
       const u64 col = g >> chunk_bits;
       const u32 idx = (u32)((g & ((1ull << chunk_bits) - 1)) << S) + (e & ((1u << S) - 1));
       x = canon(a.in[col * a.stride_in + idx]);
-      if (a.pre_scale) x = mul(x, scale_factor(a, N2B ? idx : bitrev32(idx, L), false));
+      // a bit-reversed input (single pass, S == L) is a permutation inside the chunk
+      const u32 nat = a.in_bitrev ? bitrev32(idx, L) : idx;
+      if (a.pre_scale) x = mul(x, scale_factor(a, N2B ? nat : bitrev32(nat, L), false));
+      lds[lds_pad(a.in_bitrev ? ((e >> S) << S) + nat : e)] = x;
+      return;  // next element (this is the static_for body)
     }
     lds[lds_pad(e)] = x;
   });
@@ -379,7 +408,7 @@ __global__ __launch_bounds__(THREADS) void ntt_tiny(PassArgs a) {
   u64 v[VALS];
   static_for<(1 << K)>([&](auto i_) {
     constexpr int i = decltype(i_)::value;
-    u64 x = canon(in[i]);
+    u64 x = canon(in[a.in_bitrev ? cbitrev(i, K) : i]);
     if (a.pre_scale) x = mul(x, scale_factor(a, N2B ? i : cbitrev(i, K), false));
     v[i] = x;
   });

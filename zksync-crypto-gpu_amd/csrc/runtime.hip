@@ -71,3 +71,56 @@ int ba_graph_launch(ba_graph_exec_t exec, ba_stream_t stream) {
 int ba_graph_destroy(ba_graph_exec_t exec) { BA_RET(hipGraphExecDestroy((hipGraphExec_t)exec)); }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- launch tracing
+// The reference's profiling hooks are nvtx ranges (crates/shivini/Cargo.toml:31) and CUDA-event
+// timing in criterion-cuda (crates/criterion-cuda/src/lib.rs:16-34).  Here a host thread can ask
+// the library to bracket each kernel it launches with two caller-owned events, so a harness
+// measures per-kernel durations on the launch stream without a profiler.
+#include <vector>
+
+namespace {
+struct TraceState {
+  hipEvent_t *events = nullptr;
+  uint32_t capacity = 0;
+  uint32_t count = 0;
+  bool open = false;  // a launch has started but not finished
+  std::vector<const char *> names;
+};
+thread_local TraceState g_trace;
+}  // namespace
+
+void ba_trace_before(hipStream_t stream, const char *kernel) {
+  TraceState &t = g_trace;
+  if (!t.events || t.count >= t.capacity) return;
+  (void)hipEventRecord(t.events[2 * t.count], stream);
+  t.names.push_back(kernel);
+  t.open = true;
+}
+
+void ba_trace_after(hipStream_t stream) {
+  TraceState &t = g_trace;
+  if (!t.events || !t.open) return;
+  (void)hipEventRecord(t.events[2 * t.count + 1], stream);
+  t.count++;
+  t.open = false;
+}
+
+extern "C" int ba_profile_begin(ba_event_t *events, uint32_t capacity) {
+  if (!events || capacity == 0) return BA_ERROR_INVALID_VALUE;
+  g_trace = TraceState{};
+  g_trace.events = (hipEvent_t *)events;
+  g_trace.capacity = capacity;
+  return BA_SUCCESS;
+}
+
+extern "C" int ba_profile_end(uint32_t *launches) {
+  if (launches) *launches = g_trace.count;
+  g_trace.events = nullptr;
+  g_trace.capacity = 0;
+  return BA_SUCCESS;
+}
+
+extern "C" const char *ba_profile_launch_name(uint32_t index) {
+  return index < g_trace.names.size() ? g_trace.names[index] : nullptr;
+}
