@@ -85,6 +85,8 @@ int ba_context_create(uint32_t powers_of_w_coarse_log_count, uint32_t powers_of_
                       ba_context_t *ctx);
 int ba_context_destroy(ba_context_t ctx);
 #define BA_OMEGA_LOG_ORDER 30u
+/* Largest column handled: 2^28 elements (the reference's plans end at 2^24, ntt.rs:74-169). */
+#define BA_MAX_LOG_N 28u
 
 /* ---------------------------------------------------------------- NTT
  * boojum_cuda::ntt::batch_ntt_internal (crates/boojum-cuda/src/ntt.rs:179-291).
@@ -96,8 +98,8 @@ int ba_context_destroy(ba_context_t ctx);
  *   bitrev_inputs=1, inverse=1: input bit-reversed, output natural
  * with w = omega_{2^log_n} and s = 1 if log_extension_degree == 0 else
  * 7 * omega_{2^(log_n + log_extension_degree)}^coset_index.
- * Preconditions (ntt.rs:192-200): 1 <= log_n, log_n + log_extension_degree <= 30,
- * 2^log_n <= strides, coset_index < 2^log_extension_degree when nonzero. */
+ * Preconditions (ntt.rs:192-200): 1 <= log_n <= BA_MAX_LOG_N, log_n + log_extension_degree
+ * <= 30, 2^log_n <= strides, coset_index < 2^log_extension_degree when nonzero. */
 int ba_batch_ntt_internal(const uint64_t *inputs, uint64_t *outputs, uint32_t log_n,
                           uint32_t num_ntts, uint32_t stride_between_input_arrays,
                           uint32_t stride_between_output_arrays, int bitrev_inputs, int inverse,

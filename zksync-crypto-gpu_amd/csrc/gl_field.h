@@ -37,19 +37,64 @@ __device__ __forceinline__ u64 sub(u64 a, u64 b) {
 
 __device__ __forceinline__ u64 neg(u64 a) { return a ? P - a : 0; }
 
-// Reduces lo + hi * 2^64 (any 128-bit value) to a canonical residue.
-__device__ __forceinline__ u64 reduce128(u64 lo, u64 hi) {
-  const u32 hl = (u32)hi;
-  const u32 hh = (u32)(hi >> 32);
-  // lo - hh * 2^96 == lo + hh (mod p), 2^96 == -1
-  u64 t0;
-  if (__builtin_sub_overflow(lo, (u64)hh, &t0)) t0 -= EPS;
-  // hl * 2^64 == hl * (2^32 - 1)
-  const u64 t1 = ((u64)hl << 32) - hl;
-  u64 r;
-  if (__builtin_add_overflow(t0, t1, &r)) r += EPS;
-  return canon(r);
+// Canonical residue of lo + hl (2^32 - 1) - hh  (lo < 2^64, hl, hh < 2^32), which is
+// lo + (hl + hh 2^32) 2^64 reduced with 2^64 = 2^32 - 1 and 2^96 = -1.
+//   r = hl EPS + lo with carry c (one v_mad_u64_u32), d = r - hh with borrow b, then
+//   c & !b: d + EPS (canonical, since d < hl EPS);  b & !c: d - EPS (canonical);
+//   otherwise d is exact and d >= p selects d + EPS (= d - p mod 2^64).
+// Written in inline asm to use the multiply-add's carry-out and to select with lane masks;
+// the SALU mask logic writes SCC, hence the clobber.
+__device__ __forceinline__ u64 reduce3(u64 lo, u32 hl, u32 hh) {
+  u64 r, c;
+  asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=&v"(r), "=&s"(c) : "v"(hl), "v"(lo));
+  const u32 r0 = (u32)r, r1 = (u32)(r >> 32);
+  u32 d0, d1, t0, t1, u0, u1, o0, o1;
+  u64 b, ov, x, m1, m2;
+  asm("v_sub_co_u32 %[d0], %[b], %[r0], %[hh]\n\t"
+      "v_subb_co_u32 %[d1], %[b], %[r1], 0, %[b]\n\t"
+      "v_add_co_u32 %[t0], %[ov], %[d0], -1\n\t"
+      "v_addc_co_u32 %[t1], %[ov], %[d1], 0, %[ov]\n\t"
+      "v_sub_co_u32 %[u0], %[x], %[d0], -1\n\t"
+      "v_subb_co_u32 %[u1], %[x], %[d1], 0, %[x]\n\t"
+      "s_andn2_b64 %[m1], %[c], %[b]\n\t"
+      "s_xor_b64 %[m2], %[c], %[b]\n\t"
+      "s_andn2_b64 %[m2], %[ov], %[m2]\n\t"
+      "s_or_b64 %[m1], %[m1], %[m2]\n\t"
+      "s_andn2_b64 %[m2], %[b], %[c]\n\t"
+      "v_cndmask_b32 %[o0], %[d0], %[t0], %[m1]\n\t"
+      "v_cndmask_b32 %[o1], %[d1], %[t1], %[m1]\n\t"
+      "v_cndmask_b32 %[o0], %[o0], %[u0], %[m2]\n\t"
+      "v_cndmask_b32 %[o1], %[o1], %[u1], %[m2]"
+      : [d0] "=&v"(d0), [d1] "=&v"(d1), [t0] "=&v"(t0), [t1] "=&v"(t1), [u0] "=&v"(u0),
+        [u1] "=&v"(u1), [o0] "=&v"(o0), [o1] "=&v"(o1), [b] "=&s"(b), [ov] "=&s"(ov),
+        [x] "=&s"(x), [m1] "=&s"(m1), [m2] "=&s"(m2)
+      : [r0] "v"(r0), [r1] "v"(r1), [hh] "v"(hh), [c] "s"(c)
+      : "scc");
+  return ((u64)o1 << 32) | o0;
 }
+
+// Canonical residue of lo + hl (2^32 - 1): r = hl EPS + lo with carry c; when c, r + EPS is
+// canonical (r < hl EPS); otherwise r >= p selects r + EPS (= r - p mod 2^64).
+__device__ __forceinline__ u64 reduce2(u64 lo, u32 hl) {
+  u64 r, c;
+  asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=&v"(r), "=&s"(c) : "v"(hl), "v"(lo));
+  const u32 r0 = (u32)r, r1 = (u32)(r >> 32);
+  u32 t0, t1, o0, o1;
+  u64 ov, m;
+  asm("v_add_co_u32 %[t0], %[ov], %[r0], -1\n\t"
+      "v_addc_co_u32 %[t1], %[ov], %[r1], 0, %[ov]\n\t"
+      "s_or_b64 %[m], %[c], %[ov]\n\t"
+      "v_cndmask_b32 %[o0], %[r0], %[t0], %[m]\n\t"
+      "v_cndmask_b32 %[o1], %[r1], %[t1], %[m]"
+      : [t0] "=&v"(t0), [t1] "=&v"(t1), [o0] "=&v"(o0), [o1] "=&v"(o1), [ov] "=&s"(ov),
+        [m] "=&s"(m)
+      : [r0] "v"(r0), [r1] "v"(r1), [c] "s"(c)
+      : "scc");
+  return ((u64)o1 << 32) | o0;
+}
+
+// Reduces lo + hi * 2^64 (any 128-bit value) to a canonical residue.
+__device__ __forceinline__ u64 reduce128(u64 lo, u64 hi) { return reduce3(lo, (u32)hi, (u32)(hi >> 32)); }
 
 // 64x64 -> 128 product from four 32x32 -> 64 multiply-adds (v_mad_u64_u32).
 __device__ __forceinline__ void mul_wide(u64 a, u64 b, u64 &lo, u64 &hi) {
@@ -71,30 +116,52 @@ __device__ __forceinline__ u64 mul(u64 a, u64 b) {
 
 __device__ __forceinline__ u64 sqr(u64 a) { return mul(a, a); }
 
-// x * 2^E for a compile-time E in [0, 192); x canonical.
+// x * 2^E for 0 < E < 64, x canonical: x << E is (lo, hi) with hi < 2^E.
+template <int E>
+__device__ __forceinline__ u64 shl(u64 x) {
+  static_assert(E > 0 && E < 64, "shift out of range");
+  const u64 lo = x << E, hi = x >> (64 - E);
+  if constexpr (E <= 32)
+    return reduce2(lo, (u32)hi);
+  else
+    return reduce3(lo, (u32)hi, (u32)(hi >> 32));
+}
+
+// x * 2^-M for 0 < M <= 32, x canonical: with x = q 2^M + r,
+// x 2^-M = q + r 2^(192-M) = q - (r 2^(32-M)) 2^64 = q - (r 2^(32-M)) (2^32 - 1),
+// where q < p and the subtrahend < 2^32 (2^32 - 1) < p.
+template <int M>
+__device__ __forceinline__ u64 shr(u64 x) {
+  static_assert(M > 0 && M <= 32, "shift out of range");
+  const u64 q = x >> M;
+  const u32 r = (u32)(M == 32 ? x : ((x & ((1ull << M) - 1)) << (32 - M)));
+  return sub(q, (u64)r * EPS);
+}
+
+// x * 2^E for a compile-time E in [0, 192); x canonical.  Uses 2^96 = -1 and, for
+// 64 <= E < 96, 2^E = -2^-(96-E).  `negate` reports a sign the caller must apply.
+template <int E>
+__device__ __forceinline__ u64 mul_pow2_signed(u64 x, bool &negate) {
+  static_assert(E >= 0 && E < 192, "shift exponent out of range");
+  constexpr int F = E % 96;
+  constexpr bool s1 = E >= 96;
+  if constexpr (F == 0) {
+    negate = s1;
+    return x;
+  } else if constexpr (F < 64) {
+    negate = s1;
+    return shl<F>(x);
+  } else {
+    negate = !s1;
+    return shr<96 - F>(x);
+  }
+}
+
 template <int E>
 __device__ __forceinline__ u64 mul_pow2(u64 x) {
-  static_assert(E >= 0 && E < 192, "shift exponent out of range");
-  if constexpr (E == 0) {
-    return x;
-  } else if constexpr (E >= 96) {
-    return neg(mul_pow2<E - 96>(x));  // 2^96 == -1
-  } else if constexpr (E < 64) {
-    return reduce128(x << E, x >> (64 - E));
-  } else {
-    // 64 <= E < 96, x = x1 2^32 + x0:
-    //   x1 2^(E+32) = -x1 2^(E-64)      (E + 32 >= 96)
-    //   x0 2^E      = z 2^64 with z = x0 << (E-64) < 2^64, z 2^64 == (z0 << 32) - z0 - z1
-    constexpr int S = E - 64;
-    const u64 x0 = (u32)x, x1 = x >> 32;
-    const u64 z = x0 << S;
-    const u64 z0 = (u32)z, z1 = z >> 32;
-    const u64 y = x1 << S;  // < 2^64 since x1 < 2^32 and S < 32
-    // (z0 << 32) - (z0 + z1) stays above -2^33; bring to [0, p) then subtract y
-    u64 a;
-    if (__builtin_sub_overflow(z0 << 32, z0 + z1, &a)) a -= EPS;
-    return sub(canon(a), canon(y));
-  }
+  bool negate;
+  const u64 y = mul_pow2_signed<E>(x, negate);
+  return negate ? neg(y) : y;
 }
 
 // Generic runtime a^e, for tests and set-up code.

@@ -1,0 +1,39 @@
+// gl_host.h -- host-side Goldilocks arithmetic for table and launch-constant set-up.
+#pragma once
+
+#include <stdint.h>
+
+namespace gl_host {
+
+typedef unsigned __int128 u128;
+constexpr uint64_t P = 0xFFFFFFFF00000001ull;
+
+inline uint64_t mul(uint64_t a, uint64_t b) { return (uint64_t)(((u128)a * b) % P); }
+
+inline uint64_t pow(uint64_t a, uint64_t e) {
+  uint64_t r = 1;
+  a %= P;
+  while (e) {
+    if (e & 1) r = mul(r, a);
+    a = mul(a, a);
+    e >>= 1;
+  }
+  return r;
+}
+
+inline uint64_t inv(uint64_t a) { return pow(a, P - 2); }
+
+// omega_{2^k}: the 2^32-th root 7^((p-1)/2^32) squared (32 - k) times
+inline uint64_t omega(unsigned log_size) {
+  uint64_t w = pow(7, (P - 1) >> 32);
+  for (unsigned i = log_size; i < 32; i++) w = mul(w, w);
+  return w;
+}
+
+// coset shift s = 7 omega_{2^(log_n + log_ext)}^coset_idx, or 1 without extension
+inline uint64_t coset_shift(unsigned log_n, unsigned log_ext, unsigned coset_idx) {
+  if (log_ext == 0) return 1;
+  return mul(7, pow(omega(log_n + log_ext), coset_idx));
+}
+
+}  // namespace gl_host

@@ -26,7 +26,7 @@ extern "C" int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_strid
                             uint64_t out_coset_stride, uint32_t log_n, uint32_t num_cols,
                             uint32_t log_blowup, ba_stream_t stream) {
   if (!ctx) return BA_ERROR_NOT_INITIALIZED;
-  if (log_n < 1 || log_blowup < 1 || log_n + log_blowup > gl::OMEGA_LOG_ORDER)
+  if (log_n < 1 || log_n > BA_MAX_LOG_N || log_blowup < 1 || log_n + log_blowup > gl::OMEGA_LOG_ORDER)
     return BA_ERROR_INVALID_VALUE;
   const uint64_t n = 1ull << log_n;
   const uint32_t blowup = 1u << log_blowup;

@@ -5,9 +5,11 @@
 // (ntt.rs:201-241); here every size 1..30 runs in at most 1 + ceil((log_n - 12) / 8)
 // passes of radix-16 register blocks (see ntt_kernels.h).
 #include <algorithm>
+#include <cstdio>
 
 #include "api_internal.h"
-#include "ntt_kernels.h"
+#include "gl_host.h"
+#include "ntt_inst.h"
 
 using gl::ntt::PassArgs;
 
@@ -48,47 +50,49 @@ NttPlan ba_plan_ntt(uint32_t log_n, bool n2b) {
 
 namespace {
 
-typedef void (*PassKernel)(PassArgs);
+using namespace gl::ntt;
 
-PassKernel pick_kernel(bool contig, bool n2b, bool inverse) {
-  using namespace gl::ntt;
-  if (contig) {
-    if (n2b) return inverse ? ntt_pass_contig<true, true> : ntt_pass_contig<true, false>;
-    return inverse ? ntt_pass_contig<false, true> : ntt_pass_contig<false, false>;
-  }
-  if (n2b) return inverse ? ntt_pass_strided<true, true> : ntt_pass_strided<true, false>;
-  return inverse ? ntt_pass_strided<false, true> : ntt_pass_strided<false, false>;
+BaPassKernel pick_kernel(bool contig, uint32_t S, bool n2b, bool inverse, bool top, bool scale,
+                         bool whole, bool gather) {
+  if (contig)
+    return S <= 8 ? ba_pick_contig_lo(S, n2b, inverse, scale, whole)
+                  : ba_pick_contig_hi(S, n2b, inverse, scale, whole);
+  return ba_pick_strided(S, n2b, inverse, top, scale, gather);
 }
 
-PassKernel pick_tiny(uint32_t log_n, bool n2b, bool inverse) {
-  using namespace gl::ntt;
-#define BA_TINY(K)                                                        \
-  if (log_n == K) {                                                       \
-    if (n2b) return inverse ? ntt_tiny<K, true, true> : ntt_tiny<K, true, false>; \
-    return inverse ? ntt_tiny<K, false, true> : ntt_tiny<K, false, false>;        \
+// kernel names as rocprofv3 prints them, for launch tracing
+const char *kernel_name(bool contig, uint32_t S, bool n2b, bool inverse, bool top, bool scale,
+                        bool gather) {
+  static char names[2][13][2][2][2][2][2][120];
+  static bool ready = false;
+  static const char *tf[2] = {"false", "true"};
+  if (!ready) {
+    for (int s = 0; s < 13; s++)
+      for (int d = 0; d < 2; d++)
+        for (int i = 0; i < 2; i++)
+          for (int tp = 0; tp < 2; tp++)
+            for (int sc = 0; sc < 2; sc++)
+              for (int ga = 0; ga < 2; ga++) {
+                snprintf(names[1][s][d][i][tp][sc][ga], sizeof(names[1][s][d][i][tp][sc][ga]),
+                         "void gl::ntt::ntt_pass_contig<%d, %s, %s, %s, %s>(gl::ntt::PassArgs)", s, tf[d],
+                         tf[i], tf[sc], tf[tp]);
+                snprintf(names[0][s][d][i][tp][sc][ga], sizeof(names[0][s][d][i][tp][sc][ga]),
+                         "void gl::ntt::ntt_pass_strided<%d, %s, %s, %s, %s, %s>(gl::ntt::PassArgs)", s,
+                         tf[d], tf[i], tf[tp], tf[sc], tf[ga]);
+              }
+    ready = true;
   }
-  BA_TINY(1)
-  BA_TINY(2)
-  BA_TINY(3)
-#undef BA_TINY
-  return nullptr;
+  return names[contig][S][n2b][inverse][top][scale][gather];
 }
 
 constexpr uint64_t kMaxGrid = 0x7FFFFFFFull;
 
-// names as rocprofv3 reports the kernels
-const char *kernel_name(bool contig, bool n2b, bool inverse) {
-  static const char *names[8] = {
-      "void gl::ntt::ntt_pass_strided<false, false>(gl::ntt::PassArgs)",
-      "void gl::ntt::ntt_pass_strided<false, true>(gl::ntt::PassArgs)",
-      "void gl::ntt::ntt_pass_strided<true, false>(gl::ntt::PassArgs)",
-      "void gl::ntt::ntt_pass_strided<true, true>(gl::ntt::PassArgs)",
-      "void gl::ntt::ntt_pass_contig<false, false>(gl::ntt::PassArgs)",
-      "void gl::ntt::ntt_pass_contig<false, true>(gl::ntt::PassArgs)",
-      "void gl::ntt::ntt_pass_contig<true, false>(gl::ntt::PassArgs)",
-      "void gl::ntt::ntt_pass_contig<true, true>(gl::ntt::PassArgs)",
-  };
-  return names[(contig ? 4 : 0) + (n2b ? 2 : 0) + (inverse ? 1 : 0)];
+// Columns (tile repeats) per workgroup: enough to amortise the twiddle and scale-factor set-up,
+// while keeping >= ~1024 workgroups (4 per CU) in flight.
+uint32_t pick_reps(uint64_t tiles) {
+  uint32_t reps = 1;
+  while (reps < 8 && tiles / (2ull * reps) >= 1024) reps *= 2;
+  return reps;
 }
 
 }  // namespace
@@ -101,10 +105,22 @@ int ba_launch_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint
   const bool n2b = !bitrev_inputs;
   PassArgs base{};
   base.log_n = log_n;
-  base.coset = log_ext > 0;
+  base.scale_coset = log_ext > 0;
   base.coset_mul = log_ext > 0 ? coset_idx << (gl::OMEGA_LOG_ORDER - log_n - log_ext) : 0;
   base.tab = ctx->tab;
   const bool pre = !inverse && log_ext > 0;
+  // scale table T[m] = Delta^m (times n^-1 for the inverse), Delta = s^(+-2^(L-4))
+  {
+    const uint64_t s = gl_host::coset_shift(log_n, log_ext, coset_idx);
+    const uint64_t n_inv = gl_host::inv(1ull << log_n);
+    const uint64_t sd = log_n >= 4 ? gl_host::pow(s, 1ull << (log_n - 4)) : 1;
+    const uint64_t delta = inverse ? gl_host::inv(sd) : sd;
+    uint64_t acc = inverse ? n_inv : 1;
+    for (int m = 0; m < VALS; m++) {
+      base.scale_tab[m] = acc;
+      acc = gl_host::mul(acc, delta);
+    }
+  }
 
   if (log_n < 4) {
     PassArgs a = base;
@@ -113,14 +129,11 @@ int ba_launch_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint
     a.stride_in = stride_in;
     a.stride_out = stride_out;
     a.num_cols = num_cols;
-    a.bits = log_n;
-    a.pre_scale = pre;
-    a.post_scale = inverse;
+    a.scale = pre ? SCALE_PRE : (inverse ? SCALE_POST : SCALE_NONE);
     a.in_bitrev = in_bitrev_gather;
-    const uint32_t grid = (num_cols + gl::ntt::THREADS - 1) / gl::ntt::THREADS;
+    const uint32_t grid = (num_cols + THREADS - 1) / THREADS;
     ba_trace_before(stream, "ntt_tiny");
-    hipLaunchKernelGGL(pick_tiny(log_n, n2b, inverse), dim3(grid), dim3(gl::ntt::THREADS), 0,
-                       stream, a);
+    hipLaunchKernelGGL(ba_pick_tiny(log_n, n2b, inverse), dim3(grid), dim3(THREADS), 0, stream, a);
     ba_trace_after(stream);
     return (int)hipGetLastError();
   }
@@ -129,15 +142,16 @@ int ba_launch_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint
   for (int p = 0; p < plan.num_passes; p++) {
     const uint32_t S = plan.bits[p];
     const bool contig = plan.contig[p];
-    // tiles per column (strided and large contiguous passes) or columns per tile (small)
-    const uint64_t per_col_tiles = log_n >= (uint32_t)gl::ntt::TILE_LOG
-                                       ? (1ull << (log_n - gl::ntt::TILE_LOG))
-                                       : 0;
-    const uint32_t cols_per_tile = per_col_tiles ? 1 : (1u << (gl::ntt::TILE_LOG - log_n));
-    const uint64_t max_cols = per_col_tiles ? std::max<uint64_t>(1, kMaxGrid / per_col_tiles)
-                                            : kMaxGrid * cols_per_tile;
+    // tiles: L >= 12 -> 2^(L-12) per column, repeats stride over columns;
+    //        L < 12  -> 2^(12-L) columns per tile (contiguous pass only), repeats adjacent
+    const bool big = log_n >= (uint32_t)TILE_LOG;
+    const uint32_t log_stride = big ? log_n - TILE_LOG : 0;
+    const uint32_t cols_per_tile = big ? 1 : (1u << (TILE_LOG - log_n));
+    // launch column groups small enough for 32-bit tile counts and grids
+    const uint64_t max_cols = big ? std::max<uint64_t>(1, (kMaxGrid >> log_stride)) : kMaxGrid;
     for (uint64_t c0 = 0; c0 < num_cols; c0 += max_cols) {
       const uint32_t nc = (uint32_t)std::min<uint64_t>(max_cols, num_cols - c0);
+      const uint64_t tiles = big ? ((uint64_t)nc << log_stride) : (nc + cols_per_tile - 1) / cols_per_tile;
       PassArgs a = base;
       a.in = p == 0 ? in + c0 * stride_in : out + c0 * stride_out;
       a.stride_in = p == 0 ? stride_in : stride_out;
@@ -145,14 +159,20 @@ int ba_launch_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint
       a.stride_out = stride_out;
       a.num_cols = nc;
       a.lo_bit = plan.lo_bit[p];
-      a.bits = S;
-      a.pre_scale = pre && p == 0;
-      a.post_scale = inverse && p == plan.num_passes - 1;
+      a.tile_count = (uint32_t)tiles;
+      a.log_tile_stride = log_stride;
+      a.reps = pick_reps(tiles);
+      a.scale = (pre && p == 0) ? SCALE_PRE
+                                : ((inverse && p == plan.num_passes - 1) ? SCALE_POST : SCALE_NONE);
       a.in_bitrev = in_bitrev_gather && p == 0;
-      const uint64_t grid = per_col_tiles ? per_col_tiles * nc : (nc + cols_per_tile - 1) / cols_per_tile;
-      ba_trace_before(stream, kernel_name(contig, n2b, inverse));
-      hipLaunchKernelGGL(pick_kernel(contig, n2b, inverse), dim3((uint32_t)grid),
-                         dim3(gl::ntt::THREADS), 0, stream, a);
+      const uint64_t groups = big ? (nc + a.reps - 1) / a.reps : (tiles + a.reps - 1) / a.reps;
+      const uint64_t grid = groups << log_stride;
+      const bool top = plan.lo_bit[p] + S == log_n;
+      const bool scale = a.scale != SCALE_NONE;
+      const bool gather = !contig && a.in_bitrev;
+      ba_trace_before(stream, kernel_name(contig, S, n2b, inverse, contig ? big : top, scale, gather));
+      hipLaunchKernelGGL(pick_kernel(contig, S, n2b, inverse, top, scale, big, gather), dim3((uint32_t)grid),
+                         dim3(THREADS), 0, stream, a);
       ba_trace_after(stream);
       const hipError_t err = hipGetLastError();
       if (err != hipSuccess) return (int)err;
@@ -165,8 +185,10 @@ namespace {
 
 int validate(uint32_t log_n, uint64_t stride_in, uint64_t stride_out, uint32_t log_ext,
              uint32_t coset_idx) {
-  // ntt.rs:192-200
-  if (log_n < 1 || log_n + log_ext > gl::OMEGA_LOG_ORDER) return BA_ERROR_INVALID_VALUE;
+  // ntt.rs:192-200; in addition a column holds at most 2^28 elements (the reference's plans
+  // stop at 2^24, ntt.rs:74-169, and index past them for larger sizes)
+  if (log_n < 1 || log_n > BA_MAX_LOG_N || log_n + log_ext > gl::OMEGA_LOG_ORDER)
+    return BA_ERROR_INVALID_VALUE;
   const uint64_t n = 1ull << log_n;
   if (n > stride_in || n > stride_out) return BA_ERROR_INVALID_VALUE;
   if (coset_idx > 0 && (log_ext == 0 || coset_idx >= (1u << log_ext))) return BA_ERROR_INVALID_VALUE;
@@ -208,7 +230,7 @@ extern "C" int ba_batch_ntt_out_of_place(const uint64_t *inputs_matrix, size_t i
                                          int inverse, uint32_t log_extension_degree,
                                          uint32_t coset_index, ba_stream_t stream) {
   // ntt.rs:309-320, with 64-bit arithmetic in place of the reference's u32 products
-  if (log_n < 1 || log_n > gl::OMEGA_LOG_ORDER) return BA_ERROR_INVALID_VALUE;
+  if (log_n < 1 || log_n > BA_MAX_LOG_N) return BA_ERROR_INVALID_VALUE;
   const uint64_t n = 1ull << log_n;
   if ((uint64_t)inputs_offset + n > stride_between_input_arrays) return BA_ERROR_INVALID_VALUE;
   if ((uint64_t)outputs_offset + n > stride_between_output_arrays) return BA_ERROR_INVALID_VALUE;
@@ -229,7 +251,7 @@ extern "C" int ba_batch_ntt_in_place(uint64_t *inputs_matrix, size_t inputs_len,
                                      int inverse, uint32_t log_extension_degree,
                                      uint32_t coset_index, ba_stream_t stream) {
   // ntt.rs:350-352
-  if (log_n < 1 || log_n > gl::OMEGA_LOG_ORDER) return BA_ERROR_INVALID_VALUE;
+  if (log_n < 1 || log_n > BA_MAX_LOG_N) return BA_ERROR_INVALID_VALUE;
   const uint64_t n = 1ull << log_n;
   if ((uint64_t)inputs_offset + n > stride_between_input_arrays) return BA_ERROR_INVALID_VALUE;
   if ((uint64_t)inputs_len < (uint64_t)num_ntts * stride_between_input_arrays) return BA_ERROR_INVALID_VALUE;

@@ -10,14 +10,19 @@
 // local 2^K-point transforms only use powers of omega_16, which are powers of two mod p
 // (omega_64 = 2^39), so their twiddles are shifts.
 //
-// A pass covers S <= 12 index bits with one 4096-element tile per 256-thread workgroup,
-// 16 elements per thread in registers, LDS transposes between radix-16 rounds:
+// A pass covers S <= 12 index bits (S a template parameter, so every round shape, register
+// index and LDS offset is a compile-time constant) with a 4096-element tile per 256-thread
+// workgroup, 16 elements per thread in registers, LDS transposes between radix-16 rounds:
 //   * contiguous pass: bits [0, S), each tile holds 4096/2^S contiguous chunks of 2^S;
 //   * strided pass:    bits [P0, P0 + S), S <= 8, each tile holds 2^S rows of 4096/2^S >= 16
 //                      consecutive elements (128 B+ runs), read and written in place.
+// A workgroup repeats its tile over several columns (tiles that share twiddles), computing the
+// round twiddles and scale factors once and keeping them in registers.
 // n2b runs its strided passes (high bits first) then the contiguous pass; b2n runs the
-// contiguous pass first.  The forward coset scale s^j is fused into the first pass's loads and
-// the inverse n^-1 s^-j scale into the last pass's stores.
+// contiguous pass first.  The forward coset scale s^j is fused into the first pass and the
+// inverse n^-1 s^-j scale into the last pass.  The natural indices j of a thread's 16 elements
+// there are j0 + pi(m) 2^(L-4) (pi the identity or a 4-bit reversal), so the factor is
+// F(j0) * T[pi(m)] with T[m] = (s^(+-2^(L-4)))^m (times n^-1) built on the host per launch.
 #pragma once
 
 #include <utility>
@@ -55,16 +60,17 @@ constexpr int local_exp(int m, int x, bool inv) {
 
 __device__ __forceinline__ u32 bitrev32(u32 x, u32 bits) { return bits ? (__brev(x) >> (32 - bits)) : 0; }
 
-// (a, b) <- (a + 2^E b, a - 2^E b); 2^96 = -1 turns E >= 96 into a swapped butterfly.
+// (a, b) <- (a + 2^E b, a - 2^E b); a negative shift factor (2^96 = -1, and 2^E = -2^-(96-E)
+// for 64 <= E < 96) swaps the add and the subtract instead of negating.
 template <int E>
 __device__ __forceinline__ void bfly_dit(u64 &a, u64 &b) {
   const u64 a0 = a;
-  if constexpr (E >= 96) {
-    const u64 t = mul_pow2<E - 96>(b);
+  bool negate;
+  const u64 t = mul_pow2_signed<E>(b, negate);
+  if (negate) {
     a = sub(a0, t);
     b = add(a0, t);
   } else {
-    const u64 t = mul_pow2<E>(b);
     a = add(a0, t);
     b = sub(a0, t);
   }
@@ -73,11 +79,11 @@ __device__ __forceinline__ void bfly_dit(u64 &a, u64 &b) {
 // (a, b) <- (a + b, 2^E (a - b))
 template <int E>
 __device__ __forceinline__ void bfly_dif(u64 &a, u64 &b) {
+  constexpr int F = E % 96;
+  constexpr bool flip = (E >= 96) != (F >= 64);  // sign of the factor mul_pow2_signed returns
   const u64 s = add(a, b);
-  if constexpr (E >= 96)
-    b = mul_pow2<E - 96>(sub(b, a));
-  else
-    b = mul_pow2<E>(sub(a, b));
+  bool negate;
+  b = mul_pow2_signed<E>(flip ? sub(b, a) : sub(a, b), negate);
   a = s;
 }
 
@@ -127,16 +133,20 @@ __device__ __forceinline__ void local_round(u64 (&v)[VALS]) {
   });
 }
 
-// v[m] *= W^(m >> (4-K)) for W = omega_{2^30}^(+-e)
+// W^0 .. W^(2^K - 1)
 template <int K>
-__device__ __forceinline__ void apply_twiddles(u64 (&v)[VALS], u64 w) {
-  constexpr int NW = 1 << K;
-  u64 pw[NW];
+__device__ __forceinline__ void twiddle_powers(u64 (&pw)[VALS], u64 w) {
+  pw[0] = 1;
   pw[1] = w;
-  static_for<NW>([&](auto i_) {
+  static_for<(1 << K)>([&](auto i_) {
     constexpr int i = decltype(i_)::value;
     if constexpr (i >= 2) pw[i] = (i & 1) ? mul(pw[i - 1], w) : sqr(pw[i / 2]);
   });
+}
+
+// v[m] *= W^(m >> (4-K))
+template <int K>
+__device__ __forceinline__ void apply_twiddles(u64 (&v)[VALS], const u64 (&pw)[VALS]) {
   static_for<VALS>([&](auto m_) {
     constexpr int m = decltype(m_)::value;
     constexpr int i = m >> (4 - K);
@@ -144,258 +154,368 @@ __device__ __forceinline__ void apply_twiddles(u64 (&v)[VALS], u64 w) {
   });
 }
 
+enum : u32 { SCALE_NONE = 0, SCALE_PRE = 1, SCALE_POST = 2 };
+
 struct PassArgs {
   const u64 *in;
   u64 *out;
   u64 stride_in;
   u64 stride_out;
   u32 num_cols;
-  u32 log_n;      // L
-  u32 lo_bit;     // P0, lowest index bit this pass transforms
-  u32 bits;       // S
-  u32 pre_scale;  // forward coset: inputs *= s^j
-  u32 post_scale; // inverse: outputs *= n^-1 (s^-j when coset)
-  u32 coset;      // log_extension_degree > 0
-  u32 coset_mul;  // coset_index << (30 - L - log_ext): s^j = 7^j omega_{2^30}^(j * coset_mul)
-  u32 in_bitrev;  // first pass only: natural element j is read from position bitrev_L(j)
+  u32 log_n;             // L
+  u32 lo_bit;            // P0, lowest index bit this pass transforms
+  u32 tile_count;        // tiles in the batch
+  u32 log_tile_stride;   // tiles sharing twiddles are 2^log_tile_stride apart
+  u32 reps;              // tiles (columns) per workgroup
+  u32 scale;             // SCALE_*: forward coset pre-scale / inverse post-scale
+  u32 scale_coset;       // the factor depends on j (log_extension_degree > 0)
+  u32 coset_mul;         // s^j = 7^j omega_{2^30}^(j * coset_mul)
+  u32 in_bitrev;         // first n2b pass: natural element j is read from position bitrev_L(j)
+  u64 scale_tab[VALS];   // T[m]
   Tables tab;
 };
 
-// s^j or n^-1 s^-j for natural index j
-__device__ __forceinline__ u64 scale_factor(const PassArgs &a, u32 j, bool inverse) {
-  if (!inverse) {
-    u64 f = table_pow(a.tab.g, j);
-    if (a.coset_mul) f = mul(f, pow_w(a.tab, j * a.coset_mul, false));
-    return f;
-  }
-  u64 f = a.tab.inv_sizes[a.log_n];
-  if (a.coset) {
-    f = mul(f, table_pow(a.tab.g_inv, j));
-    if (a.coset_mul) f = mul(f, pow_w(a.tab, j * a.coset_mul, true));
-  }
+// F(j0) = s^j0 (forward) or s^-j0 (inverse), or 1 without coset
+__device__ __forceinline__ u64 scale_base(const PassArgs &a, u32 j0, bool inverse) {
+  if (!a.scale_coset) return 1;
+  u64 f = table_pow(inverse ? a.tab.g_inv : a.tab.g, j0);
+  if (a.coset_mul) f = mul(f, pow_w(a.tab, j0 * a.coset_mul, inverse));
   return f;
 }
 
-// index bits of a for register m in the round whose 4-bit window starts at bit wb
-__device__ __forceinline__ u32 window_a(u32 q, u32 wb, u32 m) {
-  return ((q >> wb) << (wb + 4)) | (m << wb) | (q & ((1u << wb) - 1));
+template <bool PERM>
+__device__ __forceinline__ void scale_factors(u64 (&fac)[VALS], const PassArgs &a, u32 j0, bool inverse) {
+  const u64 f = scale_base(a, j0, inverse);
+  static_for<VALS>([&](auto m_) {
+    constexpr int m = decltype(m_)::value;
+    const u64 tm = a.scale_tab[PERM ? cbitrev(m, 4) : m];
+    fac[m] = a.scale_coset ? mul(f, tm) : tm;
+  });
+}
+
+// index bits of a for register m in the round whose 4-bit window starts at bit WB
+template <int WB>
+__device__ __forceinline__ u32 window_a(u32 q, int m) {
+  return ((q >> WB) << (WB + 4)) | ((u32)m << WB) | (q & ((1u << WB) - 1));
 }
 
 __device__ __forceinline__ u32 lds_pad(u32 u) { return u + (u >> 4); }
 
-// Runs round `r` of a pass (its window, its K, twiddles).  gidx0 = in-column index of the
-// thread's element with window bits zero.
-template <bool N2B, bool INV>
-__device__ __forceinline__ void run_round(u64 (&v)[VALS], const PassArgs &a, u32 K, u32 wb, u32 ab,
-                                          u32 gidx0) {
+// Round shapes of an S-bit pass: n2b goes from the top bits down (partial round first),
+// b2n from the bottom up (partial round last).  K = stages, WB = window base, AB = lowest
+// active bit (the active bits are the top K of the window).
+template <int S, bool N2B>
+struct Shape {
+  static constexpr int K0 = S & 3;
+  static constexpr int FULL = S >> 2;
+  static constexpr int ROUNDS = FULL + (K0 ? 1 : 0);
+  static constexpr int K(int r) {
+    if (N2B) return (K0 && r == 0) ? K0 : 4;
+    return r < FULL ? 4 : K0;
+  }
+  static constexpr int WB(int r) {
+    if (N2B) return (K0 && r == 0) ? S - 4 : S - K0 - 4 * ((K0 ? r - 1 : r) + 1);
+    return r < FULL ? 4 * r : S - 4;
+  }
+  static constexpr int AB(int r) { return WB(r) + 4 - K(r); }
+  static constexpr int TOP = N2B ? 0 : ROUNDS - 1;  // the round acting on the pass's top bits
+};
+
+// W for a round whose active bits are [P, P + K) of an L-bit index; gidx = any in-column
+// index of the thread's elements in that round.
+__device__ __forceinline__ u64 round_twiddle(const PassArgs &a, u32 gidx, u32 P, u32 K, bool inverse) {
   const u32 L = a.log_n;
-  const u32 P = a.lo_bit + ab;
-  const u32 hi_bits = L - P - K;  // bits above the active ones
-  u64 w = 1;
-  bool twiddle = false;
-  if (hi_bits) {
-    const u32 r = gidx0 >> (P + K);
-    const u32 e = bitrev32(r, hi_bits) << (OMEGA_LOG_ORDER - (L - P));
-    if (e) {
-      w = pow_w(a.tab, e, INV);
-      twiddle = true;
-    }
-  }
-  switch (K) {
-    case 4:
-      if (N2B && twiddle) apply_twiddles<4>(v, w);
-      local_round<4, N2B, INV>(v);
-      if (!N2B && twiddle) apply_twiddles<4>(v, w);
-      break;
-    case 3:
-      if (N2B && twiddle) apply_twiddles<3>(v, w);
-      local_round<3, N2B, INV>(v);
-      if (!N2B && twiddle) apply_twiddles<3>(v, w);
-      break;
-    case 2:
-      if (N2B && twiddle) apply_twiddles<2>(v, w);
-      local_round<2, N2B, INV>(v);
-      if (!N2B && twiddle) apply_twiddles<2>(v, w);
-      break;
-    default:
-      if (N2B && twiddle) apply_twiddles<1>(v, w);
-      local_round<1, N2B, INV>(v);
-      if (!N2B && twiddle) apply_twiddles<1>(v, w);
-      break;
-  }
+  const u32 hi_bits = L - P - K;
+  if (!hi_bits) return 1;
+  const u32 e = bitrev32(gidx >> (P + K), hi_bits) << (OMEGA_LOG_ORDER - (L - P));
+  return e ? pow_w(a.tab, e, inverse) : 1;
 }
 
-// round schedule: n2b goes from the top bits down, the partial (K < 4) round first;
-// b2n goes from the bottom up, the partial round last.
-__device__ __forceinline__ void round_shape(bool n2b, u32 S, u32 r, u32 &K, u32 &wb, u32 &ab) {
-  const u32 k0 = S & 3;
-  const u32 full = S >> 2;
-  if (n2b) {
-    if (k0 && r == 0) {
-      K = k0;
-      wb = S - 4;
-      ab = S - k0;
-    } else {
-      const u32 idx = k0 ? r - 1 : r;  // idx-th full window from the top
-      K = 4;
-      wb = S - k0 - 4 * (idx + 1);
-      ab = wb;
-    }
-  } else {
-    if (r < full) {
-      K = 4;
-      wb = 4 * r;
-      ab = wb;
-    } else {
-      K = k0;
-      wb = S - 4;
-      ab = S - k0;
-    }
-  }
+template <int K, bool N2B, bool INV>
+__device__ __forceinline__ void do_round(u64 (&v)[VALS], const u64 (&pw)[VALS], bool twiddle) {
+  if (N2B && twiddle) apply_twiddles<K>(v, pw);
+  local_round<K, N2B, INV>(v);
+  if (!N2B && twiddle) apply_twiddles<K>(v, pw);
 }
 
-template <bool N2B, bool INV>
+// Loads/stores address a column as a uniform base plus a 32-bit per-thread byte offset
+// (columns hold at most 2^28 elements), so the per-element addresses the column loop reuses are
+// immediate or scalar offsets rather than registers.
+__device__ __forceinline__ const u64 *at(const u64 *base, u32 byte_off) {
+  return (const u64 *)((const char *)base + byte_off);
+}
+__device__ __forceinline__ u64 *at(u64 *base, u32 byte_off) { return (u64 *)((char *)base + byte_off); }
+
+// TOP: the pass holds the index's top bits (P0 + S == L), so one round has no twiddle.
+// SCALE: fused n2b pre-scale / b2n post-scale (only the top pass is first / last).
+// GATHER: the n2b top pass reads a column stored bit-reversed.
+template <int S, bool N2B, bool INV, bool TOP, bool SCALE, bool GATHER>
 __global__ __launch_bounds__(THREADS) void ntt_pass_strided(PassArgs a) {
-  __shared__ u64 lds[TILE];
+  static_assert(TOP || !SCALE, "only the top pass scales");
+  static_assert(!GATHER || (N2B && TOP), "only the n2b top pass gathers");
+  using Sh = Shape<S, N2B>;
+  constexpr int R = Sh::ROUNDS;
+  constexpr int LOG_C = TILE_LOG - S;  // consecutive l values per tile (>= 16)
+  constexpr int C = 1 << LOG_C;
+  constexpr int ROW = C + 1;           // padded LDS row (odd stride in u64)
+  __shared__ u64 lds[(1 << S) * ROW];
   const u32 t = threadIdx.x;
-  const u32 L = a.log_n, S = a.bits, P0 = a.lo_bit;
-  const u32 log_c = TILE_LOG - S;
-  const u32 tiles_per_col = 1u << (L - TILE_LOG);
-  const u32 col = blockIdx.x >> (L - TILE_LOG);
-  const u32 tc = blockIdx.x & (tiles_per_col - 1);
-  const u32 nl = P0 - log_c;
+  const u32 L = a.log_n, P0 = a.lo_bit;
+  const u32 tc = blockIdx.x & ((1u << a.log_tile_stride) - 1);
+  const u32 grp = blockIdx.x >> a.log_tile_stride;
+  const u32 c = t & (C - 1);
+  const u32 q = t >> LOG_C;
+  const u32 nl = P0 - LOG_C;
   const u32 lblk = tc & ((1u << nl) - 1);
   const u32 h = tc >> nl;
-  const u32 c = t & ((1u << log_c) - 1);
-  const u32 q = t >> log_c;
-  const u32 base = (h << (P0 + S)) + (lblk << log_c) + c;  // in-column index of (a = 0, c)
-  const u64 *in = a.in + (u64)col * a.stride_in;
-  u64 *out = a.out + (u64)col * a.stride_out;
+  const u32 base = (h << (P0 + S)) + (lblk << LOG_C) + c;  // in-column index of (a = 0, c)
+  (void)L;
 
-  const u32 rounds = (S >> 2) + ((S & 3) ? 1 : 0);
-  u64 v[VALS];
-  u32 K, wb, ab;
-  round_shape(N2B, S, 0, K, wb, ab);
-  if (a.in_bitrev) {
-    // Top pass (P0 + S == L) over an input stored bit-reversed: the natural rows a of the
-    // tile's column l sit, permuted, in the contiguous chunk [bitrev(l) 2^S, +2^S).  Stage
-    // them through LDS with coalesced chunk reads.
-    const u32 l0 = lblk << log_c;
-    static_for<VALS>([&](auto i_) {
-      constexpr int i = decltype(i_)::value;
-      const u32 e = t + THREADS * i;
-      const u32 cl = e >> S, off = e & ((1u << S) - 1);
-      const u32 l = l0 + cl;
-      const u32 arow = bitrev32(off, S);
-      u64 x = canon(in[(bitrev32(l, P0) << S) + off]);
-      const u32 j = (arow << P0) + l;
-      if (a.pre_scale) x = mul(x, scale_factor(a, N2B ? j : bitrev32(j, L), false));
-      lds[(arow << log_c) + cl] = x;
-    });
-    __syncthreads();
-    static_for<VALS>([&](auto m_) {
-      constexpr int m = decltype(m_)::value;
-      v[m] = lds[(window_a(q, wb, m) << log_c) + c];
-    });
-    __syncthreads();
-  } else {
-    static_for<VALS>([&](auto m_) {
-      constexpr int m = decltype(m_)::value;
-      const u32 idx = base + (window_a(q, wb, m) << P0);
-      u64 x = canon(in[idx]);
-      if (a.pre_scale) x = mul(x, scale_factor(a, N2B ? idx : bitrev32(idx, L), false));
-      v[m] = x;
-    });
-  }
-  for (u32 r = 0;; r++) {
-    run_round<N2B, INV>(v, a, K, wb, ab, base + (window_a(q, wb, 0) << P0));
-    if (r + 1 == rounds) break;
-    static_for<VALS>([&](auto m_) {
-      constexpr int m = decltype(m_)::value;
-      lds[(window_a(q, wb, m) << log_c) + c] = v[m];
-    });
-    __syncthreads();
-    round_shape(N2B, S, r + 1, K, wb, ab);
-    static_for<VALS>([&](auto m_) {
-      constexpr int m = decltype(m_)::value;
-      v[m] = lds[(window_a(q, wb, m) << log_c) + c];
-    });
-    __syncthreads();
-  }
-  static_for<VALS>([&](auto m_) {
-    constexpr int m = decltype(m_)::value;
-    const u32 idx = base + (window_a(q, wb, m) << P0);
-    u64 x = v[m];
-    if (a.post_scale) x = mul(x, scale_factor(a, N2B ? bitrev32(idx, L) : idx, true));
-    out[idx] = x;
+  u64 tw[R][VALS];
+  static_for<R>([&](auto r_) {
+    constexpr int r = decltype(r_)::value;
+    if constexpr (!(TOP && r == Sh::TOP)) {
+      const u64 w = round_twiddle(a, base + (window_a<Sh::WB(r)>(q, 0) << P0), P0 + Sh::AB(r), Sh::K(r), INV);
+      twiddle_powers<Sh::K(r)>(tw[r], w);
+    }
   });
+  u64 fac[VALS];
+  // n2b pre-scale on the first (top) pass, b2n post-scale on the last (top) pass; both act
+  // on the window [S-4, S): natural j = base + a 2^P0 with a = window_a<S-4>(q, m)
+  if constexpr (SCALE) scale_factors<false>(fac, a, base + (window_a<S - 4>(q, 0) << P0), INV);
+
+  constexpr int WB0 = Sh::WB(0);
+  constexpr int WBL = Sh::WB(R - 1);
+  const u32 off0 = (base + (window_a<WB0>(q, 0) << P0)) * 8u;  // byte offset of register 0
+  const u32 offL = (base + (window_a<WBL>(q, 0) << P0)) * 8u;
+  const u32 lds0 = window_a<WB0>(q, 0) * ROW + c;
+  constexpr bool gather = GATHER;
+  const u32 l0 = lblk << LOG_C;
+  // the next column's elements are loaded while the current one is transformed
+  u64 nxt[VALS];
+  auto fetch = [&](u32 col) {
+    const u64 *in = a.in + (u64)col * a.stride_in;
+    if constexpr (gather) {
+      // top pass over a column stored bit-reversed: natural row r of column l sits at
+      // bitrev(l) 2^S + bitrev_S(r).  Each thread reads natural rows in order (permuted reads
+      // inside 2^S-element chunks); they go to consecutive padded LDS rows.
+      static_for<VALS>([&](auto k_) {
+        constexpr int k = decltype(k_)::value;
+        const u32 e = t + THREADS * k;
+        const u32 cl = e >> S, row = e & ((1u << S) - 1);
+        const u64 *chunk = in + ((u64)bitrev32(l0 + cl, P0) << S);
+        nxt[k] = chunk[bitrev32(row, S)];
+      });
+    } else {
+      static_for<VALS>([&](auto m_) {
+        constexpr int m = decltype(m_)::value;
+        nxt[m] = *at(in + ((u64)m << (WB0 + P0)), off0);
+      });
+    }
+  };
+  u32 col = grp * a.reps;
+  const u32 col_end = min(col + a.reps, a.num_cols);
+  if (col < col_end) fetch(col);
+  for (; col < col_end; col++) {
+    u64 *out = a.out + (u64)col * a.stride_out;
+    u64 v[VALS];
+    if constexpr (gather) {
+      static_for<VALS>([&](auto k_) {
+        constexpr int k = decltype(k_)::value;
+        const u32 e = t + THREADS * k;
+        lds[(e & ((1u << S) - 1)) * ROW + (e >> S)] = canon(nxt[k]);
+      });
+      __syncthreads();
+      static_for<VALS>([&](auto m_) {
+        constexpr int m = decltype(m_)::value;
+        v[m] = lds[lds0 + m * (ROW << WB0)];
+      });
+      __syncthreads();
+    } else {
+      static_for<VALS>([&](auto m_) {
+        constexpr int m = decltype(m_)::value;
+        v[m] = canon(nxt[m]);
+      });
+    }
+    if (col + 1 < col_end) fetch(col + 1);
+    if constexpr (N2B && SCALE) {
+      static_for<VALS>([&](auto m_) {
+        constexpr int m = decltype(m_)::value;
+        v[m] = mul(v[m], fac[m]);
+      });
+    }
+    static_for<R>([&](auto r_) {
+      constexpr int r = decltype(r_)::value;
+      if constexpr (r > 0) {
+        constexpr int WBP = Sh::WB(r - 1), WBR = Sh::WB(r);
+        const u32 lp = window_a<WBP>(q, 0) * ROW + c;
+        const u32 lr = window_a<WBR>(q, 0) * ROW + c;
+        static_for<VALS>([&](auto m_) {
+          constexpr int m = decltype(m_)::value;
+          lds[lp + m * (ROW << WBP)] = v[m];
+        });
+        __syncthreads();
+        static_for<VALS>([&](auto m_) {
+          constexpr int m = decltype(m_)::value;
+          v[m] = lds[lr + m * (ROW << WBR)];
+        });
+        __syncthreads();
+      }
+      do_round<Sh::K(r), N2B, INV>(v, tw[r], !(TOP && r == Sh::TOP));
+    });
+    if constexpr (!N2B && SCALE) {
+      static_for<VALS>([&](auto m_) {
+        constexpr int m = decltype(m_)::value;
+        v[m] = mul(v[m], fac[m]);
+      });
+    }
+    static_for<VALS>([&](auto m_) {
+      constexpr int m = decltype(m_)::value;
+      u64 *bm = out + ((u64)m << (WBL + P0));
+      *at(bm, offL) = v[m];
+    });
+  }
 }
 
-template <bool N2B, bool INV>
+// SCALE: fused forward pre-scale (INV = false) or inverse post-scale (INV = true).
+// WHOLE: columns of >= 4096 elements (a tile is one contiguous block of one column); otherwise
+// a tile holds 4096 / 2^S whole columns of 2^S elements.
+template <int S, bool N2B, bool INV, bool SCALE, bool WHOLE>
 __global__ __launch_bounds__(THREADS) void ntt_pass_contig(PassArgs a) {
+  using Sh = Shape<S, N2B>;
+  constexpr int R = Sh::ROUNDS;
+  constexpr int LOG_C = TILE_LOG - S;  // chunks per tile
   __shared__ u64 lds[TILE + TILE / 16];
   const u32 t = threadIdx.x;
-  const u32 L = a.log_n, S = a.bits;
-  const u32 log_c = TILE_LOG - S;
+  const u32 L = a.log_n;
   const u32 chunk_bits = L - S;  // chunks per column = 2^chunk_bits
   const u64 total_chunks = (u64)a.num_cols << chunk_bits;
-  const u64 g0 = (u64)blockIdx.x << log_c;
-
-  // coalesced load of the tile into LDS: linear element e -> chunk e >> S, offset e & (2^S-1)
-  static_for<VALS>([&](auto i_) {
-    constexpr int i = decltype(i_)::value;
-    const u32 e = t + THREADS * i;
-    const u64 g = g0 + (e >> S);
-    u64 x = 0;
-    if (g < total_chunks) {
-      const u64 col = g >> chunk_bits;
-      const u32 idx = (u32)((g & ((1ull << chunk_bits) - 1)) << S) + (e & ((1u << S) - 1));
-      x = canon(a.in[col * a.stride_in + idx]);
-      // a bit-reversed input (single pass, S == L) is a permutation inside the chunk
-      const u32 nat = a.in_bitrev ? bitrev32(idx, L) : idx;
-      if (a.pre_scale) x = mul(x, scale_factor(a, N2B ? nat : bitrev32(nat, L), false));
-      lds[lds_pad(a.in_bitrev ? ((e >> S) << S) + nat : e)] = x;
-      return;  // next element (this is the static_for body)
-    }
-    lds[lds_pad(e)] = x;
-  });
-  __syncthreads();
-
+  const u32 tc = blockIdx.x & ((1u << a.log_tile_stride) - 1);
+  const u32 grp = blockIdx.x >> a.log_tile_stride;
   const u32 q = t & ((1u << (S - 4)) - 1);
   const u32 c = t >> (S - 4);
-  const u64 my_chunk = g0 + c;
-  const u32 hbase = (u32)((my_chunk & ((1ull << chunk_bits) - 1)) << S);
-  const u32 rounds = (S >> 2) + ((S & 3) ? 1 : 0);
-  u64 v[VALS];
-  u32 K, wb, ab;
-  round_shape(N2B, S, 0, K, wb, ab);
-  for (u32 r = 0;; r++) {
-    static_for<VALS>([&](auto m_) {
-      constexpr int m = decltype(m_)::value;
-      v[m] = lds[lds_pad((c << S) + window_a(q, wb, m))];
-    });
-    __syncthreads();
-    run_round<N2B, INV>(v, a, K, wb, ab, hbase + window_a(q, wb, 0));
-    static_for<VALS>([&](auto m_) {
-      constexpr int m = decltype(m_)::value;
-      lds[lds_pad((c << S) + window_a(q, wb, m))] = v[m];
-    });
-    __syncthreads();
-    if (r + 1 == rounds) break;
-    round_shape(N2B, S, r + 1, K, wb, ab);
-  }
-  static_for<VALS>([&](auto i_) {
-    constexpr int i = decltype(i_)::value;
-    const u32 e = t + THREADS * i;
-    const u64 g = g0 + (e >> S);
-    if (g < total_chunks) {
-      const u64 col = g >> chunk_bits;
-      const u32 idx = (u32)((g & ((1ull << chunk_bits) - 1)) << S) + (e & ((1u << S) - 1));
-      u64 x = lds[lds_pad(e)];
-      if (a.post_scale) x = mul(x, scale_factor(a, N2B ? bitrev32(idx, L) : idx, true));
-      a.out[col * a.stride_out + idx] = x;
-    }
+  // chunks of the tiles this workgroup repeats over share their in-column position, hence
+  // their twiddles; hbase = in-column index of the thread's chunk start
+  const u64 tile0 = ((u64)grp * a.reps << a.log_tile_stride) + tc;
+  const u32 hbase = (u32)((((tile0 << LOG_C) + c) & ((1ull << chunk_bits) - 1)) << S);
+  // a tile of a column of >= 4096 elements is one contiguous 32 KB block of that column
+  constexpr bool whole = WHOLE;
+  const u32 tile_in_col = (u32)((tile0 << (LOG_C + S)) & ((1ull << L) - 1));  // first element
+
+  u64 tw[R][VALS];
+  static_for<R>([&](auto r_) {
+    constexpr int r = decltype(r_)::value;
+    const u64 w = round_twiddle(a, hbase + window_a<Sh::WB(r)>(q, 0), Sh::AB(r), Sh::K(r), INV);
+    twiddle_powers<Sh::K(r)>(tw[r], w);
   });
+  const bool top = S == L;  // single pass: the top round has no twiddle
+  u64 fac[VALS];
+  if constexpr (SCALE) {
+    // n2b pre (single pass) and b2n post (single pass) act on the window [S-4, S), identity order;
+    // n2b post and b2n pre act on the window [0, 4), natural j = bitrev_L(index)
+    if constexpr (N2B == !INV)
+      scale_factors<false>(fac, a, hbase + window_a<S - 4>(q, 0), INV);
+    else
+      scale_factors<true>(fac, a, bitrev32(hbase + window_a<0>(q, 0), L), INV);
+  }
+
+  // the next tile's elements are loaded while the current one is transformed
+  u64 nxt[VALS];
+  auto fetch = [&](u64 tile) {
+    if constexpr (whole) {
+      const u64 col = tile >> a.log_tile_stride;
+      const u64 *in = a.in + col * a.stride_in + tile_in_col;
+      static_for<VALS>([&](auto k_) {
+        constexpr int k = decltype(k_)::value;
+        nxt[k] = *at(in + THREADS * k, t * 8u);
+      });
+    } else {
+      const u64 g0 = tile << LOG_C;
+      static_for<VALS>([&](auto k_) {
+        constexpr int k = decltype(k_)::value;
+        const u32 e = t + THREADS * k;
+        const u64 g = g0 + (e >> S);
+        nxt[k] = g < total_chunks ? a.in[g * a.stride_in + (e & ((1u << S) - 1))] : 0;
+      });
+    }
+  };
+  const u32 reps =
+      tile0 < a.tile_count
+          ? (u32)min((u64)a.reps, (a.tile_count - tile0 + (1ull << a.log_tile_stride) - 1) >> a.log_tile_stride)
+          : 0;
+  if (reps) fetch(tile0);
+  for (u32 i = 0; i < reps; i++) {
+    const u64 tile = tile0 + ((u64)i << a.log_tile_stride);
+    const u64 g0 = tile << LOG_C;
+    static_for<VALS>([&](auto k_) {
+      constexpr int k = decltype(k_)::value;
+      const u32 e = t + THREADS * k;
+      u32 slot = e;
+      // a bit-reversed input (single pass) is a permutation inside each column
+      if (N2B && a.in_bitrev) slot = ((e >> S) << S) + bitrev32(e & ((1u << S) - 1), L);
+      lds[lds_pad(slot)] = canon(nxt[k]);
+    });
+    if (i + 1 < reps) fetch(tile + (1ull << a.log_tile_stride));
+    __syncthreads();
+    u64 v[VALS];
+    static_for<R>([&](auto r_) {
+      constexpr int r = decltype(r_)::value;
+      constexpr int WB = Sh::WB(r);
+      const u32 u0 = (c << S) + window_a<WB>(q, 0);
+      const u32 l0 = lds_pad(u0);
+      // for WB >= 4 the padding advances by 2^(WB-4) per step of m, for WB = 0 (u0 a multiple
+      // of 16) not at all; partial windows at WB = 1..3 (S = 5..7) are padded per element
+      constexpr bool LINEAR = WB == 0 || WB >= 4;
+      constexpr u32 STEP = (1u << WB) + (WB >= 4 ? (1u << (WB - 4)) : 0);
+      auto slot = [&](int m) { return LINEAR ? l0 + m * STEP : lds_pad(u0 + ((u32)m << WB)); };
+      static_for<VALS>([&](auto m_) {
+        constexpr int m = decltype(m_)::value;
+        v[m] = lds[slot(m)];
+      });
+      __syncthreads();
+      if (SCALE && !INV && r == 0) {
+        static_for<VALS>([&](auto m_) {
+          constexpr int m = decltype(m_)::value;
+          v[m] = mul(v[m], fac[m]);
+        });
+      }
+      do_round<Sh::K(r), N2B, INV>(v, tw[r], !(top && r == Sh::TOP));
+      if (SCALE && INV && r == R - 1) {
+        static_for<VALS>([&](auto m_) {
+          constexpr int m = decltype(m_)::value;
+          v[m] = mul(v[m], fac[m]);
+        });
+      }
+      static_for<VALS>([&](auto m_) {
+        constexpr int m = decltype(m_)::value;
+        lds[slot(m)] = v[m];
+      });
+      __syncthreads();
+    });
+    if constexpr (whole) {
+      const u64 col = tile >> a.log_tile_stride;
+      u64 *out = a.out + col * a.stride_out + tile_in_col;
+      static_for<VALS>([&](auto k_) {
+        constexpr int k = decltype(k_)::value;
+        *at(out + THREADS * k, t * 8u) = lds[lds_pad(t + THREADS * k)];
+      });
+    } else {
+      static_for<VALS>([&](auto k_) {
+        constexpr int k = decltype(k_)::value;
+        const u32 e = t + THREADS * k;
+        const u64 g = g0 + (e >> S);
+        if (g < total_chunks) a.out[g * a.stride_out + (e & ((1u << S) - 1))] = lds[lds_pad(e)];
+      });
+    }
+    __syncthreads();
+  }
+}
+
+// Scale factor for natural index j, computed directly (tiny transforms only).
+__device__ __forceinline__ u64 tiny_factor(const PassArgs &a, u32 j, bool inverse) {
+  u64 f = a.scale_tab[0];  // 1 (forward) or n^-1 (inverse)
+  if (a.scale_coset) f = mul(f, scale_base(a, j, inverse));
+  return f;
 }
 
 // log_n <= 3: one thread per column, the whole transform is one local block.
@@ -409,7 +529,7 @@ __global__ __launch_bounds__(THREADS) void ntt_tiny(PassArgs a) {
   static_for<(1 << K)>([&](auto i_) {
     constexpr int i = decltype(i_)::value;
     u64 x = canon(in[a.in_bitrev ? cbitrev(i, K) : i]);
-    if (a.pre_scale) x = mul(x, scale_factor(a, N2B ? i : cbitrev(i, K), false));
+    if (a.scale == SCALE_PRE) x = mul(x, tiny_factor(a, N2B ? i : cbitrev(i, K), false));
     v[i] = x;
   });
   if constexpr (N2B)
@@ -419,7 +539,7 @@ __global__ __launch_bounds__(THREADS) void ntt_tiny(PassArgs a) {
   static_for<(1 << K)>([&](auto i_) {
     constexpr int i = decltype(i_)::value;
     u64 x = v[i];
-    if (a.post_scale) x = mul(x, scale_factor(a, N2B ? cbitrev(i, K) : i, true));
+    if (a.scale == SCALE_POST) x = mul(x, tiny_factor(a, N2B ? cbitrev(i, K) : i, true));
     out[i] = x;
   });
 }
