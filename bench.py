@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--log-blowup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-per-call", action="store_true", help="skip the per-column-call leg")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a 2^27-element copy kernel (canonicalise, 8 B read + 8 B written "
                          "per element) that scales the FETCH_SIZE/WRITE_SIZE counters")
@@ -66,6 +67,78 @@ def cpu_baseline(log_n, log_blowup):
         "sample": f"{cols} columns of the same 2^{log_n} blowup-{1 << log_blowup} LDE, one per thread, "
                   f"{dt:.2f} s wall",
     }
+
+
+def run_uneven(ba, ctx, stream, seed=3, jobs_n=512, reps=5):
+    """Uneven batch: 512 single-column jobs, log_n uniform in 12..22, random forward/inverse
+    flag, 30% coset LDEs at blowup 2 or 4.  One batch plan (grouped launches) replayed as a
+    hipGraph, against one library call per job on the same stream."""
+    import ctypes
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    lib = ba.lib()
+    jobs, bufs, calls, out_elems = [], [], [], 0
+    for i in range(jobs_n):
+        log_n = int(rng.integers(12, 23))
+        n = 1 << log_n
+        if rng.random() < 0.3:
+            lb = int(rng.integers(1, 3))
+            d_in, d_mono, d_out = ba.DeviceAllocation(n), ba.DeviceAllocation(n), ba.DeviceAllocation(n << lb)
+            ba.runtime.fill_uniform(d_in, n, 1, 1000 + i)
+            jobs.append(ba.batch.lde_job(d_in, d_out, log_n, 1, lb, monomials=d_mono))
+            calls.append(("lde", d_in, d_out, d_mono, log_n, lb))
+            bufs += [d_in, d_mono, d_out]
+            out_elems += n << lb
+        else:
+            inverse = bool(rng.integers(0, 2))
+            d_in, d_out = ba.DeviceAllocation(n), ba.DeviceAllocation(n)
+            ba.runtime.fill_uniform(d_in, n, 1, 1000 + i)
+            jobs.append(ba.batch.ntt_job(d_in, d_out, log_n, 1, n, n, False, inverse))
+            calls.append(("ntt", d_in, d_out, None, log_n, inverse))
+            bufs += [d_in, d_out]
+            out_elems += n
+    plan = ba.batch.BatchPlan(ctx, jobs)
+    ba.check(lib.ba_stream_begin_capture(stream.handle), "begin_capture")
+    plan.launch(stream)
+    graph = ctypes.c_void_p(None)
+    ba.check(lib.ba_stream_end_capture(stream.handle, ctypes.byref(graph)), "end_capture")
+
+    def batched():
+        ba.check(lib.ba_graph_launch(graph, stream.handle), "graph_launch")
+
+    def per_call():
+        for kind, d_in, d_out, d_mono, log_n, x in calls:
+            n = 1 << log_n
+            if kind == "lde":
+                ba.lde.lde_batch(ctx, d_in, log_n, 1, x, d_out, monomials=d_mono, stream=stream)
+            else:
+                ba.ntt.batch_ntt_ctx(ctx, d_in.ptr, d_out.ptr, log_n, 1, n, n, False, x, stream=stream)
+
+    def timed(fn):
+        fn()
+        stream.synchronize()
+        e0, e1 = ba.CudaEvent(), ba.CudaEvent()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / reps
+        e0.destroy()
+        e1.destroy()
+        return ms
+
+    b_ms, p_ms = timed(batched), timed(per_call)
+    st = plan.stats()
+    ba.check(lib.ba_graph_destroy(graph), "graph_destroy")
+    plan.destroy()
+    for b in bufs:
+        b.free()
+    return {"jobs": jobs_n, "log_n": "12..22 uniform", "lde_fraction": 0.3, "output_elems": out_elems,
+            "batched_ms": round(b_ms, 4), "batched_melem_s": round(out_elems / (b_ms / 1e3) / 1e6, 1),
+            "per_call_ms": round(p_ms, 4), "per_call_melem_s": round(out_elems / (p_ms / 1e3) / 1e6, 1),
+            "batched_speedup": round(p_ms / b_ms, 3), "plan_launches": st["launches"], "plan_levels": st["levels"]}
 
 
 def main():
@@ -200,6 +273,8 @@ def main():
         "per_call": per_call,
         "wall_s": round(wall, 3),
     }
+    if not args.no_uneven:
+        line["uneven"] = run_uneven(ba, ctx, stream)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(log_n, lb)
     ba.check(lib.ba_graph_destroy(graph), "graph_destroy")

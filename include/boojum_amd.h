@@ -152,6 +152,38 @@ int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_stride, uint64_t
                  uint64_t *out, uint64_t out_col_stride, uint64_t out_coset_stride, uint32_t log_n,
                  uint32_t num_cols, uint32_t log_blowup, ba_stream_t stream);
 
+/* ---------------------------------------------------------------- uneven batches
+ * No reference counterpart (boojum-cuda runs one transform per call, ntt.rs:179-291): a list of
+ * independent jobs -- batch NTTs of any size, direction and coset, and coset LDEs -- planned
+ * once into a few grouped launches (one per dependency level and pass shape) and launched any
+ * number of times, e.g. inside a captured hipGraph.  Jobs must not alias each other's buffers.
+ * The plan holds device copies of the job descriptors; buffers are not owned. */
+#define BA_JOB_NTT 0u
+#define BA_JOB_LDE 1u
+typedef struct {
+  uint32_t kind;                 /* BA_JOB_NTT or BA_JOB_LDE */
+  uint32_t log_n;
+  uint32_t num_cols;
+  uint32_t bitrev_inputs;        /* NTT: as ba_batch_ntt_internal */
+  uint32_t inverse;              /* NTT */
+  uint32_t log_extension_degree; /* NTT */
+  uint32_t coset_index;          /* NTT */
+  uint32_t log_blowup;           /* LDE: as ba_lde_batch */
+  uint64_t *inputs;              /* NTT input / LDE evaluations */
+  uint64_t *outputs;             /* NTT output / LDE coset evaluations */
+  uint64_t *monomials;           /* LDE only, may be NULL */
+  uint64_t stride_in;            /* column stride of inputs */
+  uint64_t stride_out;           /* column stride of outputs */
+  uint64_t out_coset_stride;     /* LDE only */
+} ba_job;
+typedef struct ba_batch_plan *ba_batch_plan_t;
+int ba_batch_plan_create(ba_context_t ctx, const ba_job *jobs, uint32_t num_jobs,
+                         ba_batch_plan_t *plan);
+int ba_batch_plan_launch(ba_batch_plan_t plan, ba_stream_t stream);
+int ba_batch_plan_stats(ba_batch_plan_t plan, uint32_t *launches, uint32_t *levels,
+                        uint64_t *workgroups);
+int ba_batch_plan_destroy(ba_batch_plan_t plan);
+
 /* ---------------------------------------------------------------- launch tracing
  * No reference counterpart beyond its nvtx ranges (crates/shivini/Cargo.toml:31): after
  * ba_profile_begin(events, capacity) on a host thread, each kernel the library launches from

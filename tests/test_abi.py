@@ -45,3 +45,11 @@ def test_no_gpu_calls_fail_cleanly_without_device():
 
     n = boojum_amd.device_count()
     assert n >= 0
+
+
+def test_job_struct_matches_abi():
+    """ba_job is 80 bytes (8 u32 + 3 pointers + 3 u64), as batch.hip asserts."""
+    from boojum_amd._lib import Job
+
+    assert ctypes.sizeof(Job) == 80
+    assert [f[0] for f in Job._fields_][:3] == ["kind", "log_n", "num_cols"]

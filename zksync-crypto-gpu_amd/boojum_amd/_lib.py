@@ -9,6 +9,14 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 _vp = ctypes.c_void_p
 _u32, _u64, _i32, _sz = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_size_t
 
+class Job(ctypes.Structure):
+    """ba_job (include/boojum_amd.h)."""
+    _fields_ = [("kind", _u32), ("log_n", _u32), ("num_cols", _u32), ("bitrev_inputs", _u32),
+                ("inverse", _u32), ("log_extension_degree", _u32), ("coset_index", _u32),
+                ("log_blowup", _u32), ("inputs", _vp), ("outputs", _vp), ("monomials", _vp),
+                ("stride_in", _u64), ("stride_out", _u64), ("out_coset_stride", _u64)]
+
+
 # name -> argtypes (restype int unless listed in _RESTYPES)
 SIGNATURES = {
     "ba_get_device_count": [ctypes.POINTER(_i32)],
@@ -50,6 +58,10 @@ SIGNATURES = {
     "ba_bit_reverse": [_vp, _u64, _vp, _u64, _u32, _u32, _vp],
     "ba_bit_reverse_in_place": [_vp, _u64, _u32, _u32, _vp],
     "ba_lde_batch": [_vp, _vp, _u64, _vp, _vp, _u64, _u64, _u32, _u32, _u32, _vp],
+    "ba_batch_plan_create": [_vp, ctypes.POINTER(Job), _u32, ctypes.POINTER(_vp)],
+    "ba_batch_plan_launch": [_vp, _vp],
+    "ba_batch_plan_stats": [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32), ctypes.POINTER(_u64)],
+    "ba_batch_plan_destroy": [_vp],
     "ba_profile_begin": [ctypes.POINTER(_vp), _u32],
     "ba_profile_end": [ctypes.POINTER(_u32)],
     "ba_profile_launch_name": [_u32],

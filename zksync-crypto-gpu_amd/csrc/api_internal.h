@@ -6,6 +6,8 @@
 #include "../../include/boojum_amd.h"
 #include "gl_tables.h"
 
+struct NttPlan;
+
 struct ba_context {
   int device;
   uint32_t w_coarse_log;
@@ -13,6 +15,11 @@ struct ba_context {
   void *mem;
   size_t bytes;
   gl::Tables tab;
+  // autotuned choices (null / 0 = built-in defaults): the pass split of a 2^log_n transform and
+  // the columns per workgroup of pass p
+  NttPlan (*planner)(const ba_context *, uint32_t log_n, bool n2b) = nullptr;
+  uint32_t (*reps_for)(const ba_context *, uint32_t log_n, int pass, bool n2b) = nullptr;
+  void *tuning = nullptr;
 };
 
 // The most recently created live context of `device`, or null.
@@ -36,6 +43,43 @@ struct NttPlan {
 // Pass decomposition for a 2^log_n transform (n2b order: strided passes from the top bits
 // down, then the contiguous pass; b2n runs the same passes in reverse).
 NttPlan ba_plan_ntt(uint32_t log_n, bool n2b);
+
+// One pass launch of a batch NTT: the kernel pair of its shape, its arguments and grid.
+#include <vector>
+#include "ntt_inst.h"
+struct BaPassLaunch {
+  BaKernels kernels;
+  gl::ntt::PassArgs args;
+  uint64_t grid;
+  const char *name;
+  int pass;       // index of the pass inside its transform
+  uint64_t col0;  // first column of the transform this launch covers
+};
+
+// Appends the pass launches of one batch NTT (arguments validated) to `launches`.
+int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint32_t log_n,
+                 uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs,
+                 bool inverse, uint32_t log_ext, uint32_t coset_idx, bool in_bitrev_gather,
+                 std::vector<BaPassLaunch> &launches);
+// Launches them one after another on `stream` (single-batch kernels).
+int ba_run_launches(const std::vector<BaPassLaunch> &launches, hipStream_t stream);
+uint32_t ba_pick_reps(uint64_t tiles, uint32_t cols);
+
+// A pass launch tagged with its dependency level inside a batch plan: every launch at level l
+// only reads what launches at levels < l wrote.
+struct BaLeveledLaunch {
+  uint32_t level;
+  BaPassLaunch launch;
+};
+int ba_validate_lde(uint64_t *evals, uint64_t in_stride, uint64_t *monomials, uint64_t *out,
+                    uint64_t out_col_stride, uint64_t out_coset_stride, uint32_t log_n,
+                    uint32_t num_cols, uint32_t log_blowup);
+int ba_build_lde(const ba_context *ctx, uint64_t *evals, uint64_t in_stride, uint64_t *monomials,
+                 uint64_t *out, uint64_t out_col_stride, uint64_t out_coset_stride, uint32_t log_n,
+                 uint32_t num_cols, uint32_t log_blowup, std::vector<BaLeveledLaunch> &launches,
+                 uint32_t first_level);
+int ba_validate_ntt(uint32_t log_n, uint64_t stride_in, uint64_t stride_out, uint32_t log_ext,
+                    uint32_t coset_idx);
 
 // Enqueues one batch NTT (arguments already validated).  in_bitrev_gather (natural -> bitrev
 // forward only): the input holds each column in bit-reversed order and the first pass gathers

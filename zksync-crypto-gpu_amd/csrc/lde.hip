@@ -21,11 +21,9 @@ uint32_t bitrev_bits(uint32_t x, uint32_t bits) {
 
 }  // namespace
 
-extern "C" int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_stride,
-                            uint64_t *monomials, uint64_t *out, uint64_t out_col_stride,
-                            uint64_t out_coset_stride, uint32_t log_n, uint32_t num_cols,
-                            uint32_t log_blowup, ba_stream_t stream) {
-  if (!ctx) return BA_ERROR_NOT_INITIALIZED;
+int ba_validate_lde(uint64_t *evals, uint64_t in_stride, uint64_t *monomials, uint64_t *out,
+                    uint64_t out_col_stride, uint64_t out_coset_stride, uint32_t log_n,
+                    uint32_t num_cols, uint32_t log_blowup) {
   if (log_n < 1 || log_n > BA_MAX_LOG_N || log_blowup < 1 || log_n + log_blowup > gl::OMEGA_LOG_ORDER)
     return BA_ERROR_INVALID_VALUE;
   const uint64_t n = 1ull << log_n;
@@ -37,17 +35,51 @@ extern "C" int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_strid
   if (num_cols > 1 && out_col_stride < n) return BA_ERROR_INVALID_VALUE;
   if ((void *)out == (void *)evals || (monomials && (void *)out == (void *)monomials))
     return BA_ERROR_INVALID_VALUE;
-  hipStream_t s = (hipStream_t)stream;
+  return BA_SUCCESS;
+}
+
+int ba_build_lde(const ba_context *ctx, uint64_t *evals, uint64_t in_stride, uint64_t *monomials,
+                 uint64_t *out, uint64_t out_col_stride, uint64_t out_coset_stride, uint32_t log_n,
+                 uint32_t num_cols, uint32_t log_blowup, std::vector<BaLeveledLaunch> &launches,
+                 uint32_t first_level) {
+  if (num_cols == 0) return BA_SUCCESS;
+  const uint64_t n = 1ull << log_n;
+  const uint32_t blowup = 1u << log_blowup;
   uint64_t *mono = monomials ? monomials : evals;
   const uint64_t mono_stride = monomials ? n : in_stride;
   // monomials, bit-reversed: n2b inverse without coset factors
-  int rc = ba_launch_ntt(ctx, evals, mono, log_n, num_cols, in_stride, mono_stride, false, true, 0, 0, s);
+  std::vector<BaPassLaunch> ls;
+  int rc = ba_build_ntt(ctx, evals, mono, log_n, num_cols, in_stride, mono_stride, false, true, 0, 0, false, ls);
   if (rc) return rc;
+  uint32_t depth = 0;
+  for (const BaPassLaunch &l : ls) {
+    launches.push_back({first_level + (uint32_t)l.pass, l});
+    depth = std::max<uint32_t>(depth, (uint32_t)l.pass + 1);
+  }
   for (uint32_t c = 0; c < blowup; c++) {
-    rc = ba_launch_ntt(ctx, mono, out + (uint64_t)c * out_coset_stride, log_n, num_cols, mono_stride,
-                       out_col_stride, false, false, log_blowup, bitrev_bits(c, log_blowup), s,
-                       /*in_bitrev_gather=*/true);
+    ls.clear();
+    rc = ba_build_ntt(ctx, mono, out + (uint64_t)c * out_coset_stride, log_n, num_cols, mono_stride,
+                      out_col_stride, false, false, log_blowup, bitrev_bits(c, log_blowup),
+                      /*in_bitrev_gather=*/true, ls);
     if (rc) return rc;
+    for (const BaPassLaunch &l : ls) launches.push_back({first_level + depth + (uint32_t)l.pass, l});
   }
   return BA_SUCCESS;
+}
+
+extern "C" int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_stride,
+                            uint64_t *monomials, uint64_t *out, uint64_t out_col_stride,
+                            uint64_t out_coset_stride, uint32_t log_n, uint32_t num_cols,
+                            uint32_t log_blowup, ba_stream_t stream) {
+  if (!ctx) return BA_ERROR_NOT_INITIALIZED;
+  int rc = ba_validate_lde(evals, in_stride, monomials, out, out_col_stride, out_coset_stride, log_n,
+                           num_cols, log_blowup);
+  if (rc || num_cols == 0) return rc;
+  std::vector<BaLeveledLaunch> leveled;
+  rc = ba_build_lde(ctx, evals, in_stride, monomials, out, out_col_stride, out_coset_stride, log_n,
+                    num_cols, log_blowup, leveled, 0);
+  if (rc) return rc;
+  std::vector<BaPassLaunch> flat;
+  for (const BaLeveledLaunch &l : leveled) flat.push_back(l.launch);  // already in dependency order
+  return ba_run_launches(flat, (hipStream_t)stream);
 }

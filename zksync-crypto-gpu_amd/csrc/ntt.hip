@@ -6,6 +6,7 @@
 // passes of radix-16 register blocks (see ntt_kernels.h).
 #include <algorithm>
 #include <cstdio>
+#include <vector>
 
 #include "api_internal.h"
 #include "gl_host.h"
@@ -52,8 +53,8 @@ namespace {
 
 using namespace gl::ntt;
 
-BaPassKernel pick_kernel(bool contig, uint32_t S, bool n2b, bool inverse, bool top, bool scale,
-                         bool whole, bool gather) {
+BaKernels pick_kernel(bool contig, uint32_t S, bool n2b, bool inverse, bool top, bool scale,
+                      bool whole, bool gather) {
   if (contig)
     return S <= 8 ? ba_pick_contig_lo(S, n2b, inverse, scale, whole)
                   : ba_pick_contig_hi(S, n2b, inverse, scale, whole);
@@ -87,20 +88,20 @@ const char *kernel_name(bool contig, uint32_t S, bool n2b, bool inverse, bool to
 
 constexpr uint64_t kMaxGrid = 0x7FFFFFFFull;
 
+}  // namespace
+
 // Columns (tile repeats) per workgroup: enough to amortise the twiddle and scale-factor set-up,
 // while keeping >= ~1024 workgroups (4 per CU) in flight.
-uint32_t pick_reps(uint64_t tiles) {
+uint32_t ba_pick_reps(uint64_t tiles, uint32_t cols) {
   uint32_t reps = 1;
-  while (reps < 8 && tiles / (2ull * reps) >= 1024) reps *= 2;
+  while (reps < 8 && reps < cols && tiles / (2ull * reps) >= 1024) reps *= 2;
   return reps;
 }
 
-}  // namespace
-
-int ba_launch_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint32_t log_n,
-                  uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs,
-                  bool inverse, uint32_t log_ext, uint32_t coset_idx, hipStream_t stream,
-                  bool in_bitrev_gather) {
+int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint32_t log_n,
+                 uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs,
+                 bool inverse, uint32_t log_ext, uint32_t coset_idx, bool in_bitrev_gather,
+                 std::vector<BaPassLaunch> &launches) {
   if (num_cols == 0) return BA_SUCCESS;
   const bool n2b = !bitrev_inputs;
   PassArgs base{};
@@ -123,22 +124,25 @@ int ba_launch_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint
   }
 
   if (log_n < 4) {
-    PassArgs a = base;
-    a.in = in;
-    a.out = out;
-    a.stride_in = stride_in;
-    a.stride_out = stride_out;
-    a.num_cols = num_cols;
-    a.scale = pre ? SCALE_PRE : (inverse ? SCALE_POST : SCALE_NONE);
-    a.in_bitrev = in_bitrev_gather;
-    const uint32_t grid = (num_cols + THREADS - 1) / THREADS;
-    ba_trace_before(stream, "ntt_tiny");
-    hipLaunchKernelGGL(ba_pick_tiny(log_n, n2b, inverse), dim3(grid), dim3(THREADS), 0, stream, a);
-    ba_trace_after(stream);
-    return (int)hipGetLastError();
+    BaPassLaunch l;
+    l.args = base;
+    l.args.in = in;
+    l.args.out = out;
+    l.args.stride_in = stride_in;
+    l.args.stride_out = stride_out;
+    l.args.num_cols = num_cols;
+    l.args.scale = pre ? SCALE_PRE : (inverse ? SCALE_POST : SCALE_NONE);
+    l.args.in_bitrev = in_bitrev_gather;
+    l.kernels = ba_pick_tiny(log_n, n2b, inverse);
+    l.grid = (num_cols + THREADS - 1) / THREADS;
+    l.name = "ntt_tiny";
+    l.pass = 0;
+    l.col0 = 0;
+    launches.push_back(l);
+    return BA_SUCCESS;
   }
 
-  const NttPlan plan = ba_plan_ntt(log_n, n2b);
+  const NttPlan plan = ctx->planner ? ctx->planner(ctx, log_n, n2b) : ba_plan_ntt(log_n, n2b);
   for (int p = 0; p < plan.num_passes; p++) {
     const uint32_t S = plan.bits[p];
     const bool contig = plan.contig[p];
@@ -152,7 +156,9 @@ int ba_launch_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint
     for (uint64_t c0 = 0; c0 < num_cols; c0 += max_cols) {
       const uint32_t nc = (uint32_t)std::min<uint64_t>(max_cols, num_cols - c0);
       const uint64_t tiles = big ? ((uint64_t)nc << log_stride) : (nc + cols_per_tile - 1) / cols_per_tile;
-      PassArgs a = base;
+      BaPassLaunch l;
+      PassArgs &a = l.args;
+      a = base;
       a.in = p == 0 ? in + c0 * stride_in : out + c0 * stride_out;
       a.stride_in = p == 0 ? stride_in : stride_out;
       a.out = out + c0 * stride_out;
@@ -161,30 +167,53 @@ int ba_launch_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint
       a.lo_bit = plan.lo_bit[p];
       a.tile_count = (uint32_t)tiles;
       a.log_tile_stride = log_stride;
-      a.reps = pick_reps(tiles);
+      const uint32_t tuned = ctx->reps_for ? ctx->reps_for(ctx, log_n, p, n2b) : 0;
+      a.reps = tuned ? std::min<uint32_t>(tuned, big ? nc : (uint32_t)tiles)
+                     : ba_pick_reps(tiles, big ? nc : (uint32_t)tiles);
+      if (a.reps == 0) a.reps = 1;
       a.scale = (pre && p == 0) ? SCALE_PRE
                                 : ((inverse && p == plan.num_passes - 1) ? SCALE_POST : SCALE_NONE);
       a.in_bitrev = in_bitrev_gather && p == 0;
       const uint64_t groups = big ? (nc + a.reps - 1) / a.reps : (tiles + a.reps - 1) / a.reps;
-      const uint64_t grid = groups << log_stride;
+      l.grid = groups << log_stride;
       const bool top = plan.lo_bit[p] + S == log_n;
       const bool scale = a.scale != SCALE_NONE;
       const bool gather = !contig && a.in_bitrev;
-      ba_trace_before(stream, kernel_name(contig, S, n2b, inverse, contig ? big : top, scale, gather));
-      hipLaunchKernelGGL(pick_kernel(contig, S, n2b, inverse, top, scale, big, gather), dim3((uint32_t)grid),
-                         dim3(THREADS), 0, stream, a);
-      ba_trace_after(stream);
-      const hipError_t err = hipGetLastError();
-      if (err != hipSuccess) return (int)err;
+      l.kernels = pick_kernel(contig, S, n2b, inverse, top, scale, big, gather);
+      l.name = kernel_name(contig, S, n2b, inverse, contig ? big : top, scale, gather);
+      l.pass = p;
+      l.col0 = c0;
+      if (!l.kernels.single) return BA_ERROR_INVALID_VALUE;
+      launches.push_back(l);
     }
   }
   return BA_SUCCESS;
 }
 
-namespace {
+int ba_run_launches(const std::vector<BaPassLaunch> &launches, hipStream_t stream) {
+  for (const BaPassLaunch &l : launches) {
+    ba_trace_before(stream, l.name);
+    hipLaunchKernelGGL(l.kernels.single, dim3((uint32_t)l.grid), dim3(THREADS), 0, stream, l.args);
+    ba_trace_after(stream);
+    const hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return (int)err;
+  }
+  return BA_SUCCESS;
+}
 
-int validate(uint32_t log_n, uint64_t stride_in, uint64_t stride_out, uint32_t log_ext,
-             uint32_t coset_idx) {
+int ba_launch_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint32_t log_n,
+                  uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs,
+                  bool inverse, uint32_t log_ext, uint32_t coset_idx, hipStream_t stream,
+                  bool in_bitrev_gather) {
+  std::vector<BaPassLaunch> launches;
+  const int rc = ba_build_ntt(ctx, in, out, log_n, num_cols, stride_in, stride_out, bitrev_inputs,
+                              inverse, log_ext, coset_idx, in_bitrev_gather, launches);
+  if (rc) return rc;
+  return ba_run_launches(launches, stream);
+}
+
+int ba_validate_ntt(uint32_t log_n, uint64_t stride_in, uint64_t stride_out, uint32_t log_ext,
+                    uint32_t coset_idx) {
   // ntt.rs:192-200; in addition a column holds at most 2^28 elements (the reference's plans
   // stop at 2^24, ntt.rs:74-169, and index past them for larger sizes)
   if (log_n < 1 || log_n > BA_MAX_LOG_N || log_n + log_ext > gl::OMEGA_LOG_ORDER)
@@ -195,15 +224,13 @@ int validate(uint32_t log_n, uint64_t stride_in, uint64_t stride_out, uint32_t l
   return BA_SUCCESS;
 }
 
-}  // namespace
-
 extern "C" int ba_batch_ntt_ctx(ba_context_t ctx, const uint64_t *inputs, uint64_t *outputs,
                                 uint32_t log_n, uint32_t num_ntts, uint64_t stride_in,
                                 uint64_t stride_out, int bitrev_inputs, int inverse,
                                 uint32_t log_extension_degree, uint32_t coset_index,
                                 ba_stream_t stream) {
   if (!ctx) return BA_ERROR_NOT_INITIALIZED;
-  int rc = validate(log_n, stride_in, stride_out, log_extension_degree, coset_index);
+  int rc = ba_validate_ntt(log_n, stride_in, stride_out, log_extension_degree, coset_index);
   if (rc) return rc;
   if (num_ntts && (!inputs || !outputs)) return BA_ERROR_INVALID_VALUE;
   return ba_launch_ntt(ctx, inputs, outputs, log_n, num_ntts, stride_in, stride_out,

@@ -3,17 +3,22 @@
 
 using namespace gl::ntt;
 
+template <int S, bool N2B, bool INV, bool SCALE, bool WHOLE>
+static BaKernels pair() {
+  return {ntt_pass_contig<S, N2B, INV, SCALE, WHOLE>, ntt_pass_contig_group<S, N2B, INV, SCALE, WHOLE>};
+}
+
 // columns of >= 4096 elements use S = 12, or S = 9..11 when log_n is 13..15; shorter
 // columns are transformed in one pass with S = log_n
 template <int S, bool N2B, bool INV, bool SCALE>
-static BaPassKernel pick4(bool whole) {
-  if constexpr (S == 12) return ntt_pass_contig<S, N2B, INV, SCALE, true>;
-  else if constexpr (S < 9) return ntt_pass_contig<S, N2B, INV, SCALE, false>;
-  else return whole ? ntt_pass_contig<S, N2B, INV, SCALE, true> : ntt_pass_contig<S, N2B, INV, SCALE, false>;
+static BaKernels pick4(bool whole) {
+  if constexpr (S == 12) return pair<S, N2B, INV, SCALE, true>();
+  else if constexpr (S < 9) return pair<S, N2B, INV, SCALE, false>();
+  else return whole ? pair<S, N2B, INV, SCALE, true>() : pair<S, N2B, INV, SCALE, false>();
 }
 
 template <int S>
-static BaPassKernel pick(bool n2b, bool inverse, bool scale, bool whole) {
+static BaKernels pick(bool n2b, bool inverse, bool scale, bool whole) {
   if (n2b) {
     if (inverse) return scale ? pick4<S, true, true, true>(whole) : pick4<S, true, true, false>(whole);
     return scale ? pick4<S, true, false, true>(whole) : pick4<S, true, false, false>(whole);
@@ -22,7 +27,7 @@ static BaPassKernel pick(bool n2b, bool inverse, bool scale, bool whole) {
   return scale ? pick4<S, false, false, true>(whole) : pick4<S, false, false, false>(whole);
 }
 
-BaPassKernel ba_pick_contig_lo(uint32_t S, bool n2b, bool inverse, bool scale, bool whole) {
+BaKernels ba_pick_contig_lo(uint32_t S, bool n2b, bool inverse, bool scale, bool whole) {
   switch (S) {
     case 4: return pick<4>(n2b, inverse, scale, whole);
     case 5: return pick<5>(n2b, inverse, scale, whole);
@@ -30,5 +35,5 @@ BaPassKernel ba_pick_contig_lo(uint32_t S, bool n2b, bool inverse, bool scale, b
     case 7: return pick<7>(n2b, inverse, scale, whole);
     case 8: return pick<8>(n2b, inverse, scale, whole);
   }
-  return nullptr;
+  return {nullptr, nullptr};
 }

@@ -171,9 +171,20 @@ struct PassArgs {
   u32 scale_coset;       // the factor depends on j (log_extension_degree > 0)
   u32 coset_mul;         // s^j = 7^j omega_{2^30}^(j * coset_mul)
   u32 in_bitrev;         // first n2b pass: natural element j is read from position bitrev_L(j)
+  // column k starts at in_cols[k] / out_cols[k] when these lists are given (columns gathered
+  // from separate buffers), else at in + k * stride_in / out + k * stride_out
+  const u64 *const *in_cols;
+  u64 *const *out_cols;
   u64 scale_tab[VALS];   // T[m]
   Tables tab;
 };
+
+__device__ __forceinline__ const u64 *col_in(const PassArgs &a, u64 col) {
+  return a.in_cols ? a.in_cols[col] : a.in + col * a.stride_in;
+}
+__device__ __forceinline__ u64 *col_out(const PassArgs &a, u64 col) {
+  return a.out_cols ? a.out_cols[col] : a.out + col * a.stride_out;
+}
 
 // F(j0) = s^j0 (forward) or s^-j0 (inverse), or 1 without coset
 __device__ __forceinline__ u64 scale_base(const PassArgs &a, u32 j0, bool inverse) {
@@ -250,7 +261,7 @@ __device__ __forceinline__ u64 *at(u64 *base, u32 byte_off) { return (u64 *)((ch
 // SCALE: fused n2b pre-scale / b2n post-scale (only the top pass is first / last).
 // GATHER: the n2b top pass reads a column stored bit-reversed.
 template <int S, bool N2B, bool INV, bool TOP, bool SCALE, bool GATHER>
-__global__ __launch_bounds__(THREADS) void ntt_pass_strided(PassArgs a) {
+__device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
   static_assert(TOP || !SCALE, "only the top pass scales");
   static_assert(!GATHER || (N2B && TOP), "only the n2b top pass gathers");
   using Sh = Shape<S, N2B>;
@@ -261,8 +272,8 @@ __global__ __launch_bounds__(THREADS) void ntt_pass_strided(PassArgs a) {
   __shared__ u64 lds[(1 << S) * ROW];
   const u32 t = threadIdx.x;
   const u32 L = a.log_n, P0 = a.lo_bit;
-  const u32 tc = blockIdx.x & ((1u << a.log_tile_stride) - 1);
-  const u32 grp = blockIdx.x >> a.log_tile_stride;
+  const u32 tc = bid & ((1u << a.log_tile_stride) - 1);
+  const u32 grp = bid >> a.log_tile_stride;
   const u32 c = t & (C - 1);
   const u32 q = t >> LOG_C;
   const u32 nl = P0 - LOG_C;
@@ -294,7 +305,7 @@ __global__ __launch_bounds__(THREADS) void ntt_pass_strided(PassArgs a) {
   // the next column's elements are loaded while the current one is transformed
   u64 nxt[VALS];
   auto fetch = [&](u32 col) {
-    const u64 *in = a.in + (u64)col * a.stride_in;
+    const u64 *in = col_in(a, col);
     if constexpr (gather) {
       // top pass over a column stored bit-reversed: natural row r of column l sits at
       // bitrev(l) 2^S + bitrev_S(r).  Each thread reads natural rows in order (permuted reads
@@ -317,7 +328,7 @@ __global__ __launch_bounds__(THREADS) void ntt_pass_strided(PassArgs a) {
   const u32 col_end = min(col + a.reps, a.num_cols);
   if (col < col_end) fetch(col);
   for (; col < col_end; col++) {
-    u64 *out = a.out + (u64)col * a.stride_out;
+    u64 *out = col_out(a, col);
     u64 v[VALS];
     if constexpr (gather) {
       static_for<VALS>([&](auto k_) {
@@ -381,7 +392,7 @@ __global__ __launch_bounds__(THREADS) void ntt_pass_strided(PassArgs a) {
 // WHOLE: columns of >= 4096 elements (a tile is one contiguous block of one column); otherwise
 // a tile holds 4096 / 2^S whole columns of 2^S elements.
 template <int S, bool N2B, bool INV, bool SCALE, bool WHOLE>
-__global__ __launch_bounds__(THREADS) void ntt_pass_contig(PassArgs a) {
+__device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
   using Sh = Shape<S, N2B>;
   constexpr int R = Sh::ROUNDS;
   constexpr int LOG_C = TILE_LOG - S;  // chunks per tile
@@ -390,8 +401,8 @@ __global__ __launch_bounds__(THREADS) void ntt_pass_contig(PassArgs a) {
   const u32 L = a.log_n;
   const u32 chunk_bits = L - S;  // chunks per column = 2^chunk_bits
   const u64 total_chunks = (u64)a.num_cols << chunk_bits;
-  const u32 tc = blockIdx.x & ((1u << a.log_tile_stride) - 1);
-  const u32 grp = blockIdx.x >> a.log_tile_stride;
+  const u32 tc = bid & ((1u << a.log_tile_stride) - 1);
+  const u32 grp = bid >> a.log_tile_stride;
   const u32 q = t & ((1u << (S - 4)) - 1);
   const u32 c = t >> (S - 4);
   // chunks of the tiles this workgroup repeats over share their in-column position, hence
@@ -424,7 +435,7 @@ __global__ __launch_bounds__(THREADS) void ntt_pass_contig(PassArgs a) {
   auto fetch = [&](u64 tile) {
     if constexpr (whole) {
       const u64 col = tile >> a.log_tile_stride;
-      const u64 *in = a.in + col * a.stride_in + tile_in_col;
+      const u64 *in = col_in(a, col) + tile_in_col;
       static_for<VALS>([&](auto k_) {
         constexpr int k = decltype(k_)::value;
         nxt[k] = *at(in + THREADS * k, t * 8u);
@@ -435,7 +446,7 @@ __global__ __launch_bounds__(THREADS) void ntt_pass_contig(PassArgs a) {
         constexpr int k = decltype(k_)::value;
         const u32 e = t + THREADS * k;
         const u64 g = g0 + (e >> S);
-        nxt[k] = g < total_chunks ? a.in[g * a.stride_in + (e & ((1u << S) - 1))] : 0;
+        nxt[k] = g < total_chunks ? col_in(a, g)[e & ((1u << S) - 1)] : 0;
       });
     }
   };
@@ -494,7 +505,7 @@ __global__ __launch_bounds__(THREADS) void ntt_pass_contig(PassArgs a) {
     });
     if constexpr (whole) {
       const u64 col = tile >> a.log_tile_stride;
-      u64 *out = a.out + col * a.stride_out + tile_in_col;
+      u64 *out = col_out(a, col) + tile_in_col;
       static_for<VALS>([&](auto k_) {
         constexpr int k = decltype(k_)::value;
         *at(out + THREADS * k, t * 8u) = lds[lds_pad(t + THREADS * k)];
@@ -504,7 +515,7 @@ __global__ __launch_bounds__(THREADS) void ntt_pass_contig(PassArgs a) {
         constexpr int k = decltype(k_)::value;
         const u32 e = t + THREADS * k;
         const u64 g = g0 + (e >> S);
-        if (g < total_chunks) a.out[g * a.stride_out + (e & ((1u << S) - 1))] = lds[lds_pad(e)];
+        if (g < total_chunks) col_out(a, g)[e & ((1u << S) - 1)] = lds[lds_pad(e)];
       });
     }
     __syncthreads();
@@ -520,11 +531,11 @@ __device__ __forceinline__ u64 tiny_factor(const PassArgs &a, u32 j, bool invers
 
 // log_n <= 3: one thread per column, the whole transform is one local block.
 template <int K, bool N2B, bool INV>
-__global__ __launch_bounds__(THREADS) void ntt_tiny(PassArgs a) {
-  const u32 col = blockIdx.x * THREADS + threadIdx.x;
+__device__ __forceinline__ void tiny_body(const PassArgs &a, const u32 bid) {
+  const u32 col = bid * THREADS + threadIdx.x;
   if (col >= a.num_cols) return;
-  const u64 *in = a.in + (u64)col * a.stride_in;
-  u64 *out = a.out + (u64)col * a.stride_out;
+  const u64 *in = col_in(a, col);
+  u64 *out = col_out(a, col);
   u64 v[VALS];
   static_for<(1 << K)>([&](auto i_) {
     constexpr int i = decltype(i_)::value;
@@ -542,6 +553,50 @@ __global__ __launch_bounds__(THREADS) void ntt_tiny(PassArgs a) {
     if (a.scale == SCALE_POST) x = mul(x, tiny_factor(a, N2B ? cbitrev(i, K) : i, true));
     out[i] = x;
   });
+}
+
+// Kernels: a single batch (PassArgs by value) or a group of batches sharing one pass shape, where
+// workgroup b runs batch j with wg_start[j] <= b < wg_start[j + 1] (its block index rebased).
+__device__ __forceinline__ u32 group_of(const u32 *wg_start, u32 njobs, u32 b) {
+  u32 lo = 0, hi = njobs;  // wg_start is ascending; find the last entry <= b
+  while (hi - lo > 1) {
+    const u32 mid = (lo + hi) >> 1;
+    if (wg_start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <int S, bool N2B, bool INV, bool TOP, bool SCALE, bool GATHER>
+__global__ __launch_bounds__(THREADS) void ntt_pass_strided(PassArgs a) {
+  strided_body<S, N2B, INV, TOP, SCALE, GATHER>(a, blockIdx.x);
+}
+template <int S, bool N2B, bool INV, bool TOP, bool SCALE, bool GATHER>
+__global__ __launch_bounds__(THREADS) void ntt_pass_strided_group(const PassArgs *jobs, const u32 *wg_start,
+                                                                  u32 njobs) {
+  const u32 j = group_of(wg_start, njobs, blockIdx.x);
+  strided_body<S, N2B, INV, TOP, SCALE, GATHER>(jobs[j], blockIdx.x - wg_start[j]);
+}
+template <int S, bool N2B, bool INV, bool SCALE, bool WHOLE>
+__global__ __launch_bounds__(THREADS) void ntt_pass_contig(PassArgs a) {
+  contig_body<S, N2B, INV, SCALE, WHOLE>(a, blockIdx.x);
+}
+template <int S, bool N2B, bool INV, bool SCALE, bool WHOLE>
+__global__ __launch_bounds__(THREADS) void ntt_pass_contig_group(const PassArgs *jobs, const u32 *wg_start,
+                                                                 u32 njobs) {
+  const u32 j = group_of(wg_start, njobs, blockIdx.x);
+  contig_body<S, N2B, INV, SCALE, WHOLE>(jobs[j], blockIdx.x - wg_start[j]);
+}
+template <int K, bool N2B, bool INV>
+__global__ __launch_bounds__(THREADS) void ntt_tiny(PassArgs a) {
+  tiny_body<K, N2B, INV>(a, blockIdx.x);
+}
+template <int K, bool N2B, bool INV>
+__global__ __launch_bounds__(THREADS) void ntt_tiny_group(const PassArgs *jobs, const u32 *wg_start, u32 njobs) {
+  const u32 j = group_of(wg_start, njobs, blockIdx.x);
+  tiny_body<K, N2B, INV>(jobs[j], blockIdx.x - wg_start[j]);
 }
 
 }  // namespace ntt
