@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-per-call", action="store_true", help="skip the per-column-call leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch leg")
+    ap.add_argument("--no-tune", action="store_true", help="use the built-in pass plans, no autotuning")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a 2^27-element copy kernel (canonicalise, 8 B read + 8 B written "
                          "per element) that scales the FETCH_SIZE/WRITE_SIZE counters")
@@ -156,6 +157,15 @@ def main():
     ba.set_device(0)
     ctx = ba.Context.create(12, 12)
     log_n, cols, lb = args.log_n, args.cols, args.log_blowup
+    tuning = None
+    if not args.no_tune:
+        # pass split and columns per workgroup for every size class the legs below run
+        lo, hi = min(12, log_n), max(22, log_n)
+        t0 = time.perf_counter()
+        ba.tune.autotune(ctx, lo, hi, elems_per_trial=1 << 26)
+        tuning = {"log_n_range": [lo, hi], "seconds": round(time.perf_counter() - t0, 2),
+                  f"plan_{log_n}_natural": ba.tune.get_plan(ctx, log_n, False),
+                  f"plan_{log_n}_bitrev": ba.tune.get_plan(ctx, log_n, True)}
     n, B = 1 << log_n, 1 << lb
     d_ev = ba.DeviceAllocation(n * cols)
     d_mono = ba.DeviceAllocation(n * cols)
@@ -271,6 +281,7 @@ def main():
                      "alg_bytes_per_launch": dom_bytes},
         "per_kernel_ms": {k: round(v[0] / v[1], 4) for k, v in per_kernel.items()},
         "per_call": per_call,
+        "tuning": tuning,
         "wall_s": round(wall, 3),
     }
     if not args.no_uneven:

@@ -184,6 +184,31 @@ int ba_batch_plan_stats(ba_batch_plan_t plan, uint32_t *launches, uint32_t *leve
                         uint64_t *workgroups);
 int ba_batch_plan_destroy(ba_batch_plan_t plan);
 
+/* ---------------------------------------------------------------- autotuning
+ * No reference counterpart (boojum-cuda's pass shapes are fixed tables, ntt.rs:57-176).  A size
+ * class is (log_n, input order); its plan is the split of the log_n index bits into passes --
+ * bits[0] for the contiguous pass over the lowest bits, then the strided passes upwards -- and,
+ * per pass in the same order, the columns each workgroup repeats over (reps, 1..8; 0 = built-in
+ * choice).  Valid splits: log_n <= 12 one pass of log_n bits; log_n > 12 a contiguous pass of
+ * 9..12 bits and strided passes of 4..8 bits (at most 8 passes).
+ *
+ * ba_autotune times every valid split, then each pass's reps, for log_n in [log_n_min,
+ * log_n_max] and both input orders, on batches of max(elems_per_trial, 2^log_n) elements of
+ * scratch memory it allocates and frees, and keeps the fastest on `ctx`; every later transform
+ * (NTT, LDE, batch plan) on `ctx` uses it.  Results are bit-identical for every plan.  Tuning
+ * state is per context and must not change while other threads launch with that context. */
+typedef struct {
+  uint32_t num_passes;
+  uint32_t bits[8];
+  uint32_t reps[8];
+} ba_tuned_plan;
+int ba_autotune(ba_context_t ctx, uint32_t log_n_min, uint32_t log_n_max, uint64_t elems_per_trial,
+                ba_stream_t stream);
+/* the plan in use for (log_n, bitrev_inputs): tuned if set, else the built-in one */
+int ba_tuning_get(ba_context_t ctx, uint32_t log_n, int bitrev_inputs, ba_tuned_plan *plan);
+/* sets (plan != NULL, validated) or clears (plan == NULL) the tuned plan of a size class */
+int ba_tuning_set(ba_context_t ctx, uint32_t log_n, int bitrev_inputs, const ba_tuned_plan *plan);
+
 /* ---------------------------------------------------------------- launch tracing
  * No reference counterpart beyond its nvtx ranges (crates/shivini/Cargo.toml:31): after
  * ba_profile_begin(events, capacity) on a host thread, each kernel the library launches from

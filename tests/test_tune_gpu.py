@@ -1,0 +1,124 @@
+"""GPU parity of tuned plans (ba_autotune / ba_tuning_get / ba_tuning_set): every pass split
+and columns-per-workgroup choice the autotuner may pick gives results bit-identical to the
+oracle, and invalid plans are rejected."""
+import numpy as np
+import pytest
+
+import gl_oracle
+from gpu_util import log, raw_u64
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ba():
+    import boojum_amd
+    assert boojum_amd.device_count() >= 1, "no GPU visible"
+    ctx = boojum_amd.Context.create(12, 12)
+    boojum_amd._test_ctx = ctx
+    yield boojum_amd
+    for log_n in range(1, 29):
+        for br in (False, True):
+            boojum_amd.tune.set_plan(ctx, log_n, br, None)
+    ctx.destroy()
+
+
+def run_ntt(ba, x, log_n, cols, bitrev, inverse, le=0, ci=0):
+    n = 1 << log_n
+    d_in = ba.DeviceAllocation.from_host(x)
+    d_out = ba.DeviceAllocation(cols * n)
+    try:
+        ba.ntt.batch_ntt_ctx(ba._test_ctx, d_in.ptr, d_out.ptr, log_n, cols, n, n, bitrev, inverse, le, ci)
+        ba.synchronize()
+        return d_out.to_host()
+    finally:
+        d_in.free()
+        d_out.free()
+
+
+def test_default_plans(ba):
+    t = ba.tune
+    assert t.get_plan(ba._test_ctx, 20) == ([12, 8], [0, 0])
+    assert t.get_plan(ba._test_ctx, 13, True) == ([9, 4], [0, 0])
+    assert t.get_plan(ba._test_ctx, 5) == ([5], [0])
+    assert t.get_plan(ba._test_ctx, 28) == ([12, 8, 8], [0, 0, 0])
+
+
+@pytest.mark.parametrize("log_n", [13, 14, 16, 20, 21])
+def test_every_split_bit_exact(ba, log_n):
+    """Each split the autotuner can choose, with reps cycling through 0/1/2/4/8, in both input
+    orders, forward with a coset and inverse, against the oracle."""
+    cols = 3
+    n = 1 << log_n
+    rng = np.random.default_rng(log_n)
+    x = raw_u64(rng, cols * n)
+    splits = ba.tune.candidate_splits(log_n)
+    assert splits, log_n
+    modes = [(False, False, 1, 1), (True, True, 0, 0), (False, True, 2, 3), (True, False, 0, 0)]
+    wants = {m: gl_oracle.batch_ntt(x, log_n, cols, n, n, m[0], m[1], m[2], m[3], nthreads=8) for m in modes}
+    reps_cycle = [0, 1, 2, 4, 8]
+    for k, bits in enumerate(splits):
+        reps = [reps_cycle[(k + i) % 5] for i in range(len(bits))]
+        log(f"split log_n={log_n} bits={bits} reps={reps}")
+        for m in modes:
+            ba.tune.set_plan(ba._test_ctx, log_n, m[0], bits, reps)
+            assert ba.tune.get_plan(ba._test_ctx, log_n, m[0]) == (bits, reps)
+            got = run_ntt(ba, x, log_n, cols, *m)
+            assert np.array_equal(got, wants[m]), (bits, reps, m)
+    for br in (False, True):
+        ba.tune.set_plan(ba._test_ctx, log_n, br, None)
+
+
+def test_set_plan_rejects_invalid(ba):
+    t = ba.tune
+    bad = [
+        (20, [12, 7]),        # does not cover 20 bits
+        (20, [8, 8, 4]),      # contiguous pass below 9 bits for log_n > 12
+        (20, [11, 9]),        # strided pass above 8 bits
+        (20, [12, 3, 5]),     # strided pass below 4 bits
+        (10, [6, 4]),         # log_n <= 12 is one pass
+        (13, [13]),           # contiguous pass above 12 bits
+    ]
+    for log_n, bits in bad:
+        with pytest.raises(ba.BoojumAmdError):
+            t.set_plan(ba._test_ctx, log_n, False, bits)
+    with pytest.raises(ba.BoojumAmdError):
+        t.set_plan(ba._test_ctx, 20, False, [12, 8], [9, 0])  # reps above 8
+    for log_n in (0, 29):
+        with pytest.raises(ba.BoojumAmdError):
+            t.get_plan(ba._test_ctx, log_n)
+    # a rejected plan leaves the size class unchanged
+    assert t.get_plan(ba._test_ctx, 20) == ([12, 8], [0, 0])
+
+
+def test_autotune_then_bit_exact(ba):
+    """Tune 10..17 on small trial batches, then NTTs and an LDE of those sizes still match."""
+    ba.tune.autotune(ba._test_ctx, 10, 17, elems_per_trial=1 << 20)
+    for log_n in range(10, 18):
+        for br in (False, True):
+            bits, reps = ba.tune.get_plan(ba._test_ctx, log_n, br)
+            log(f"tuned log_n={log_n} bitrev={br}: bits={bits} reps={reps}")
+            assert sum(bits) == log_n and bits in ba.tune.candidate_splits(log_n)
+            assert all(r in (0, 1, 2, 4, 8) for r in reps)
+    rng = np.random.default_rng(3)
+    for log_n in (10, 13, 17):
+        cols, n = 2, 1 << log_n
+        x = raw_u64(rng, cols * n)
+        for m in [(False, False, 1, 0), (True, True, 0, 0)]:
+            want = gl_oracle.batch_ntt(x, log_n, cols, n, n, m[0], m[1], m[2], m[3], nthreads=8)
+            assert np.array_equal(run_ntt(ba, x, log_n, cols, *m), want), (log_n, m)
+        d_ev = ba.DeviceAllocation.from_host(x)
+        d_out = ba.DeviceAllocation(cols * 4 * n)
+        try:
+            ba.lde.lde_batch(ba._test_ctx, d_ev, log_n, cols, 2, d_out)
+            ba.synchronize()
+            assert np.array_equal(d_out.to_host(), gl_oracle.lde(x, log_n, cols, 2, nthreads=8)), log_n
+        finally:
+            d_ev.free()
+            d_out.free()
+
+
+def test_autotune_rejects_bad_range(ba):
+    for lo, hi in [(0, 5), (9, 8), (20, 29)]:
+        with pytest.raises(ba.BoojumAmdError):
+            ba.tune.autotune(ba._test_ctx, lo, hi, elems_per_trial=1 << 16)

@@ -17,6 +17,11 @@ class Job(ctypes.Structure):
                 ("stride_in", _u64), ("stride_out", _u64), ("out_coset_stride", _u64)]
 
 
+class TunedPlan(ctypes.Structure):
+    """ba_tuned_plan (include/boojum_amd.h)."""
+    _fields_ = [("num_passes", _u32), ("bits", _u32 * 8), ("reps", _u32 * 8)]
+
+
 # name -> argtypes (restype int unless listed in _RESTYPES)
 SIGNATURES = {
     "ba_get_device_count": [ctypes.POINTER(_i32)],
@@ -62,6 +67,9 @@ SIGNATURES = {
     "ba_batch_plan_launch": [_vp, _vp],
     "ba_batch_plan_stats": [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32), ctypes.POINTER(_u64)],
     "ba_batch_plan_destroy": [_vp],
+    "ba_autotune": [_vp, _u32, _u32, _u64, _vp],
+    "ba_tuning_get": [_vp, _u32, _i32, ctypes.POINTER(TunedPlan)],
+    "ba_tuning_set": [_vp, _u32, _i32, ctypes.POINTER(TunedPlan)],
     "ba_profile_begin": [ctypes.POINTER(_vp), _u32],
     "ba_profile_end": [ctypes.POINTER(_u32)],
     "ba_profile_launch_name": [_u32],

@@ -6,7 +6,21 @@
 #include "../../include/boojum_amd.h"
 #include "gl_tables.h"
 
-struct NttPlan;
+// Launch-time plan of one NTT: the index bits each pass transforms, in execution order.
+struct NttPlan {
+  int num_passes;
+  uint32_t lo_bit[8];
+  uint32_t bits[8];
+  bool contig[8];
+};
+
+// An autotuned (or caller-set) choice for one size class: the pass split and, per pass in
+// execution order, the columns (tiles) each workgroup repeats over (0 = built-in choice).
+struct BaTuned {
+  bool valid;
+  NttPlan plan;
+  uint32_t reps[8];
+};
 
 struct ba_context {
   int device;
@@ -15,11 +29,7 @@ struct ba_context {
   void *mem;
   size_t bytes;
   gl::Tables tab;
-  // autotuned choices (null / 0 = built-in defaults): the pass split of a 2^log_n transform and
-  // the columns per workgroup of pass p
-  NttPlan (*planner)(const ba_context *, uint32_t log_n, bool n2b) = nullptr;
-  uint32_t (*reps_for)(const ba_context *, uint32_t log_n, int pass, bool n2b) = nullptr;
-  void *tuning = nullptr;
+  BaTuned tuned[2][BA_MAX_LOG_N + 1] = {};  // [n2b][log_n]
 };
 
 // The most recently created live context of `device`, or null.
@@ -32,17 +42,14 @@ inline ba_context *ba_context_for_current_device() {
   return ba_current_context(device);
 }
 
-// Launch-time plan of one NTT: the index bits each pass transforms, in execution order.
-struct NttPlan {
-  int num_passes;
-  uint32_t lo_bit[8];
-  uint32_t bits[8];
-  bool contig[8];
-};
-
 // Pass decomposition for a 2^log_n transform (n2b order: strided passes from the top bits
 // down, then the contiguous pass; b2n runs the same passes in reverse).
 NttPlan ba_plan_ntt(uint32_t log_n, bool n2b);
+// The plan whose passes cover bits[0] (contiguous pass, lowest bits), bits[1], ... upwards;
+// false when no kernel shape covers that split.
+bool ba_plan_from_bits(uint32_t log_n, bool n2b, const uint32_t *bits, int num_passes, NttPlan &plan);
+// The plan a transform of this size class runs with on `ctx` (tuned or built-in).
+NttPlan ba_plan_for(const ba_context *ctx, uint32_t log_n, bool n2b);
 
 // One pass launch of a batch NTT: the kernel pair of its shape, its arguments and grid.
 #include <vector>

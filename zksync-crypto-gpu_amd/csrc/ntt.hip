@@ -14,39 +14,67 @@
 
 using gl::ntt::PassArgs;
 
+bool ba_plan_from_bits(uint32_t log_n, bool n2b, const uint32_t *bits, int num_passes, NttPlan &plan) {
+  constexpr uint32_t T = gl::ntt::TILE_LOG;
+  if (num_passes < 1 || num_passes > 8) return false;
+  if (log_n < 4) {  // whole transform per thread
+    if (num_passes != 1 || bits[0] != log_n) return false;
+  } else if (log_n <= T) {  // one contiguous pass
+    if (num_passes != 1 || bits[0] != log_n) return false;
+  } else {
+    // contiguous pass over 9..12 low bits (whole 4096-element tiles), strided passes of 4..8
+    if (num_passes < 2 || bits[0] < 9 || bits[0] > T) return false;
+    uint32_t sum = bits[0];
+    for (int i = 1; i < num_passes; i++) {
+      if (bits[i] < 4 || bits[i] > 8) return false;
+      sum += bits[i];
+    }
+    if (sum != log_n) return false;
+  }
+  plan = NttPlan{};
+  plan.num_passes = num_passes;
+  uint32_t lo[8];
+  uint32_t at = 0;
+  for (int i = 0; i < num_passes; i++) lo[i] = at, at += bits[i];
+  for (int i = 0; i < num_passes; i++) {
+    const int src = n2b ? num_passes - 1 - i : i;  // n2b: top bits first
+    plan.lo_bit[i] = lo[src];
+    plan.bits[i] = bits[src];
+    plan.contig[i] = src == 0;
+  }
+  return true;
+}
+
 NttPlan ba_plan_ntt(uint32_t log_n, bool n2b) {
-  NttPlan plan{};
-  // passes listed from the low bits up
-  uint32_t lo[8], bits[8];
-  bool contig[8];
+  // passes listed from the low bits up: a contiguous pass of 12 bits, then strided passes of
+  // 8 bits, the last of 4..8 (bits borrowed from the pass below when fewer than 4 remain)
+  uint32_t bits[8];
   int np = 0;
   if (log_n <= (uint32_t)gl::ntt::TILE_LOG) {
-    lo[np] = 0, bits[np] = log_n, contig[np] = true, np++;
+    bits[np++] = log_n;
   } else {
     uint32_t rest = log_n - gl::ntt::TILE_LOG;
     uint32_t sc = gl::ntt::TILE_LOG;
-    if (rest < 4) {  // the strided pass needs >= 4 bits: borrow them from the contiguous pass
+    if (rest < 4) {
       sc -= 4 - rest;
       rest = 4;
     }
-    lo[np] = 0, bits[np] = sc, contig[np] = true, np++;
-    uint32_t at = sc;
+    bits[np++] = sc;
     while (rest > 0) {
       uint32_t take = rest;
       if (rest > 8) take = (rest - 8 >= 4) ? 8 : rest - 4;
-      lo[np] = at, bits[np] = take, contig[np] = false, np++;
-      at += take;
+      bits[np++] = take;
       rest -= take;
     }
   }
-  plan.num_passes = np;
-  for (int i = 0; i < np; i++) {
-    const int src = n2b ? np - 1 - i : i;  // n2b: top bits first
-    plan.lo_bit[i] = lo[src];
-    plan.bits[i] = bits[src];
-    plan.contig[i] = contig[src];
-  }
+  NttPlan plan{};
+  (void)ba_plan_from_bits(log_n, n2b, bits, np, plan);
   return plan;
+}
+
+NttPlan ba_plan_for(const ba_context *ctx, uint32_t log_n, bool n2b) {
+  const BaTuned &t = ctx->tuned[n2b][log_n];
+  return t.valid ? t.plan : ba_plan_ntt(log_n, n2b);
 }
 
 namespace {
@@ -142,7 +170,8 @@ int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint3
     return BA_SUCCESS;
   }
 
-  const NttPlan plan = ctx->planner ? ctx->planner(ctx, log_n, n2b) : ba_plan_ntt(log_n, n2b);
+  const BaTuned &tuning = ctx->tuned[n2b][log_n];
+  const NttPlan plan = tuning.valid ? tuning.plan : ba_plan_ntt(log_n, n2b);
   for (int p = 0; p < plan.num_passes; p++) {
     const uint32_t S = plan.bits[p];
     const bool contig = plan.contig[p];
@@ -167,7 +196,7 @@ int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint3
       a.lo_bit = plan.lo_bit[p];
       a.tile_count = (uint32_t)tiles;
       a.log_tile_stride = log_stride;
-      const uint32_t tuned = ctx->reps_for ? ctx->reps_for(ctx, log_n, p, n2b) : 0;
+      const uint32_t tuned = tuning.valid ? tuning.reps[p] : 0;
       a.reps = tuned ? std::min<uint32_t>(tuned, big ? nc : (uint32_t)tiles)
                      : ba_pick_reps(tiles, big ? nc : (uint32_t)tiles);
       if (a.reps == 0) a.reps = 1;

@@ -477,7 +477,7 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
       // for WB >= 4 the padding advances by 2^(WB-4) per step of m, for WB = 0 (u0 a multiple
       // of 16) not at all; partial windows at WB = 1..3 (S = 5..7) are padded per element
       constexpr bool LINEAR = WB == 0 || WB >= 4;
-      constexpr u32 STEP = (1u << WB) + (WB >= 4 ? (1u << (WB - 4)) : 0);
+      constexpr u32 STEP = (1u << WB) + (WB >= 4 ? (1u << (WB >= 4 ? WB - 4 : 0)) : 0);
       auto slot = [&](int m) { return LINEAR ? l0 + m * STEP : lds_pad(u0 + ((u32)m << WB)); };
       static_for<VALS>([&](auto m_) {
         constexpr int m = decltype(m_)::value;

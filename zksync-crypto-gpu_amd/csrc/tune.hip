@@ -1,0 +1,203 @@
+// tune.hip -- the autotuner: per size class (log_n, input order) it times every pass split and
+// then the columns-per-workgroup of each pass, and keeps the fastest on the context.
+//
+// No reference counterpart: boojum-cuda's kernel shapes per log_n are fixed tables
+// (crates/boojum-cuda/src/ntt.rs:57-176).  The candidate splits are those the pass kernels
+// cover: one contiguous pass for log_n <= 12; otherwise a contiguous pass over 9..12 low bits
+// (whole 4096-element tiles) below strided passes of 4..8 bits each, from the fewest strided
+// passes that cover the remaining bits to one more than that (an extra pass adds HBM traffic
+// but can pay for itself when the passes are bound by integer ALU work).
+#include <algorithm>
+#include <vector>
+
+#include "api_internal.h"
+
+namespace {
+
+// every composition of `rest` into `parts` pieces of 4..8 bits
+void compositions(uint32_t rest, int parts, std::vector<uint32_t> &cur,
+                  std::vector<std::vector<uint32_t>> &out) {
+  if (parts == 0) {
+    if (rest == 0) out.push_back(cur);
+    return;
+  }
+  for (uint32_t b = 4; b <= 8 && b <= rest; b++) {
+    cur.push_back(b);
+    compositions(rest - b, parts - 1, cur, out);
+    cur.pop_back();
+  }
+}
+
+// candidate splits, bits listed from the low (contiguous) pass up
+std::vector<std::vector<uint32_t>> candidate_splits(uint32_t log_n) {
+  std::vector<std::vector<uint32_t>> out;
+  if (log_n <= (uint32_t)gl::ntt::TILE_LOG) {
+    out.push_back({log_n});
+    return out;
+  }
+  for (uint32_t sc = 9; sc <= (uint32_t)gl::ntt::TILE_LOG; sc++) {
+    if (log_n < sc + 4) continue;
+    const uint32_t rest = log_n - sc;
+    const int min_parts = (int)((rest + 7) / 8);
+    for (int parts = min_parts; parts <= min_parts + 1 && parts <= 7; parts++) {
+      std::vector<std::vector<uint32_t>> comps;
+      std::vector<uint32_t> cur;
+      compositions(rest, parts, cur, comps);
+      for (auto &c : comps) {
+        std::vector<uint32_t> split{sc};
+        split.insert(split.end(), c.begin(), c.end());
+        out.push_back(split);
+      }
+    }
+  }
+  return out;
+}
+
+struct Timer {
+  hipStream_t stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  explicit Timer(hipStream_t s) : stream(s) {}
+  ~Timer() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+  int init() {
+    hipError_t err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+    return (int)err;
+  }
+  // mean ms per run of `ls` over enough repeats to fill ~1 ms, best of two groups
+  int time(const std::vector<BaPassLaunch> &ls, float &ms) {
+    int rc = ba_run_launches(ls, stream);  // warm-up, and a first estimate
+    if (rc) return rc;
+    hipError_t err = hipEventRecord(e0, stream);
+    if (err == hipSuccess) rc = ba_run_launches(ls, stream);
+    if (err == hipSuccess && rc == 0) err = hipEventRecord(e1, stream);
+    if (err == hipSuccess && rc == 0) err = hipEventSynchronize(e1);
+    float one = 0;
+    if (err == hipSuccess && rc == 0) err = hipEventElapsedTime(&one, e0, e1);
+    if (err != hipSuccess || rc) return rc ? rc : (int)err;
+    const int reps = std::max(2, std::min(50, (int)(1.0f / std::max(one, 1e-3f))));
+    float best = 1e30f;
+    for (int g = 0; g < 2; g++) {
+      err = hipEventRecord(e0, stream);
+      for (int i = 0; err == hipSuccess && rc == 0 && i < reps; i++) rc = ba_run_launches(ls, stream);
+      if (err == hipSuccess && rc == 0) err = hipEventRecord(e1, stream);
+      if (err == hipSuccess && rc == 0) err = hipEventSynchronize(e1);
+      float t = 0;
+      if (err == hipSuccess && rc == 0) err = hipEventElapsedTime(&t, e0, e1);
+      if (err != hipSuccess || rc) return rc ? rc : (int)err;
+      best = std::min(best, t / reps);
+    }
+    ms = best;
+    return BA_SUCCESS;
+  }
+};
+
+int time_entry(ba_context *ctx, const BaTuned &entry, uint32_t log_n, bool n2b, const uint64_t *in, uint64_t *out,
+               uint32_t cols, Timer &timer, float &ms) {
+  ctx->tuned[n2b][log_n] = entry;
+  const uint64_t n = 1ull << log_n;
+  std::vector<BaPassLaunch> ls;
+  const int rc = ba_build_ntt(ctx, in, out, log_n, cols, n, n, !n2b, false, 0, 0, false, ls);
+  if (rc) return rc;
+  return timer.time(ls, ms);
+}
+
+int tune_class(ba_context *ctx, uint32_t log_n, bool n2b, const uint64_t *in, uint64_t *out, uint64_t elems,
+               Timer &timer) {
+  const uint32_t cols = (uint32_t)std::max<uint64_t>(1, elems >> log_n);
+  BaTuned best{};
+  float best_ms = 1e30f;
+  for (const auto &split : candidate_splits(log_n)) {
+    BaTuned e{};
+    e.valid = ba_plan_from_bits(log_n, n2b, split.data(), (int)split.size(), e.plan);
+    if (!e.valid) return BA_ERROR_INVALID_VALUE;  // the enumeration only yields valid splits
+    float ms = 0;
+    const int rc = time_entry(ctx, e, log_n, n2b, in, out, cols, timer, ms);
+    if (rc) return rc;
+    if (ms < best_ms) best_ms = ms, best = e;
+  }
+  // then the columns per workgroup of each pass in turn (0 = built-in choice, already timed)
+  for (int p = 0; p < best.plan.num_passes; p++) {
+    for (uint32_t r : {1u, 2u, 4u, 8u}) {
+      BaTuned e = best;
+      e.reps[p] = r;
+      float ms = 0;
+      const int rc = time_entry(ctx, e, log_n, n2b, in, out, cols, timer, ms);
+      if (rc) return rc;
+      if (ms < best_ms * 0.99f) best_ms = ms, best = e;
+    }
+  }
+  ctx->tuned[n2b][log_n] = best;
+  return BA_SUCCESS;
+}
+
+bool context_ok(ba_context_t ctx) {
+  int device = -1;
+  return ctx && hipGetDevice(&device) == hipSuccess && device == ctx->device;
+}
+
+}  // namespace
+
+extern "C" int ba_autotune(ba_context_t ctx, uint32_t log_n_min, uint32_t log_n_max, uint64_t elems_per_trial,
+                           ba_stream_t stream) {
+  if (!ctx) return BA_ERROR_NOT_INITIALIZED;
+  if (!context_ok(ctx) || log_n_min < 1 || log_n_min > log_n_max || log_n_max > BA_MAX_LOG_N)
+    return BA_ERROR_INVALID_VALUE;
+  const uint64_t elems = std::max<uint64_t>(elems_per_trial, 1ull << log_n_max);
+  uint64_t *in = nullptr, *out = nullptr;
+  hipError_t err = hipMalloc(&in, elems * 8);
+  if (err == hipSuccess) err = hipMalloc(&out, elems * 8);
+  int rc = (int)err;
+  const auto saved = std::vector<BaTuned>(&ctx->tuned[0][0], &ctx->tuned[0][0] + 2 * (BA_MAX_LOG_N + 1));
+  if (rc == BA_SUCCESS) rc = ba_fill_uniform(in, elems, elems, 1, 0x7475u, stream);
+  {
+    Timer timer((hipStream_t)stream);
+    if (rc == BA_SUCCESS) rc = timer.init();
+    for (uint32_t log_n = log_n_min; rc == BA_SUCCESS && log_n <= log_n_max; log_n++)
+      for (int n2b = 1; rc == BA_SUCCESS && n2b >= 0; n2b--) rc = tune_class(ctx, log_n, n2b != 0, in, out, elems, timer);
+  }
+  if (rc != BA_SUCCESS) std::copy(saved.begin(), saved.end(), &ctx->tuned[0][0]);  // keep the old state
+  (void)hipStreamSynchronize((hipStream_t)stream);
+  if (in) (void)hipFree(in);
+  if (out) (void)hipFree(out);
+  return rc;
+}
+
+extern "C" int ba_tuning_get(ba_context_t ctx, uint32_t log_n, int bitrev_inputs, ba_tuned_plan *plan) {
+  if (!ctx) return BA_ERROR_NOT_INITIALIZED;
+  if (!plan || log_n < 1 || log_n > BA_MAX_LOG_N) return BA_ERROR_INVALID_VALUE;
+  const bool n2b = !bitrev_inputs;
+  const BaTuned &t = ctx->tuned[n2b][log_n];
+  const NttPlan p = t.valid ? t.plan : ba_plan_ntt(log_n, n2b);
+  *plan = ba_tuned_plan{};
+  plan->num_passes = (uint32_t)p.num_passes;
+  for (int i = 0; i < p.num_passes; i++) {
+    const int low = n2b ? p.num_passes - 1 - i : i;  // execution order -> low-bits-up order
+    plan->bits[low] = p.bits[i];
+    plan->reps[low] = t.valid ? t.reps[i] : 0;
+  }
+  return BA_SUCCESS;
+}
+
+extern "C" int ba_tuning_set(ba_context_t ctx, uint32_t log_n, int bitrev_inputs, const ba_tuned_plan *plan) {
+  if (!ctx) return BA_ERROR_NOT_INITIALIZED;
+  if (log_n < 1 || log_n > BA_MAX_LOG_N) return BA_ERROR_INVALID_VALUE;
+  const bool n2b = !bitrev_inputs;
+  if (!plan) {
+    ctx->tuned[n2b][log_n] = BaTuned{};
+    return BA_SUCCESS;
+  }
+  if (plan->num_passes < 1 || plan->num_passes > 8) return BA_ERROR_INVALID_VALUE;
+  BaTuned e{};
+  if (!ba_plan_from_bits(log_n, n2b, plan->bits, (int)plan->num_passes, e.plan)) return BA_ERROR_INVALID_VALUE;
+  for (uint32_t i = 0; i < plan->num_passes; i++) {
+    if (plan->reps[i] > 8) return BA_ERROR_INVALID_VALUE;
+    const int exec = n2b ? (int)plan->num_passes - 1 - (int)i : (int)i;
+    e.reps[exec] = plan->reps[i];
+  }
+  e.valid = true;
+  ctx->tuned[n2b][log_n] = e;
+  return BA_SUCCESS;
+}
