@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-per-call", action="store_true", help="skip the per-column-call leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-tune", action="store_true", help="use the built-in pass plans, no autotuning")
+    ap.add_argument("--no-large", action="store_true", help="skip the 256 x 2^22 blowup-4 leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serve-mode leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a 2^27-element copy kernel (canonicalise, 8 B read + 8 B written "
                          "per element) that scales the FETCH_SIZE/WRITE_SIZE counters")
@@ -140,6 +142,99 @@ def run_uneven(ba, ctx, stream, seed=3, jobs_n=512, reps=5):
             "batched_ms": round(b_ms, 4), "batched_melem_s": round(out_elems / (b_ms / 1e3) / 1e6, 1),
             "per_call_ms": round(p_ms, 4), "per_call_melem_s": round(out_elems / (p_ms / 1e3) / 1e6, 1),
             "batched_speedup": round(p_ms / b_ms, 3), "plan_launches": st["launches"], "plan_levels": st["levels"]}
+
+
+def run_large(ba, ctx, stream, log_n=22, cols=256, lb=2, steps=3):
+    """Large trace LDE: 256 columns of 2^22 at blowup 4 (2^32 output elements, 34.4 GB), the
+    whole pipeline captured as one hipGraph, evaluations kept (monomials in their own buffer)."""
+    import ctypes
+
+    lib = ba.lib()
+    n, B = 1 << log_n, 1 << lb
+    d_ev, d_mono, d_out = ba.DeviceAllocation(n * cols), ba.DeviceAllocation(n * cols), ba.DeviceAllocation(B * n * cols)
+    try:
+        ba.runtime.fill_uniform(d_ev, n, cols, 22)
+        ba.check(lib.ba_stream_begin_capture(stream.handle), "begin_capture")
+        ba.lde.lde_batch(ctx, d_ev, log_n, cols, lb, d_out, monomials=d_mono, stream=stream)
+        graph = ctypes.c_void_p(None)
+        ba.check(lib.ba_stream_end_capture(stream.handle, ctypes.byref(graph)), "end_capture")
+        ba.check(lib.ba_graph_launch(graph, stream.handle), "graph_launch")  # warm-up
+        stream.synchronize()
+        e0, e1 = ba.CudaEvent(), ba.CudaEvent()
+        e0.record(stream)
+        for _ in range(steps):
+            ba.check(lib.ba_graph_launch(graph, stream.handle), "graph_launch")
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / steps
+        e0.destroy()
+        e1.destroy()
+        ba.check(lib.ba_graph_destroy(graph), "graph_destroy")
+    finally:
+        for t in (d_ev, d_mono, d_out):
+            t.free()
+    out_elems = cols * B * n
+    alg = 8 * n * (1 + B) * cols
+    return {"workload": f"coset LDE, {cols} columns x 2^{log_n}, blowup {B}, one hipGraph",
+            "output_bytes": out_elems * 8, "steps": steps, "ms_per_step": round(ms, 3),
+            "melem_s": round(out_elems / (ms / 1e3) / 1e6, 1),
+            "lde_alg_GBs": round(alg / (ms / 1e3) / 1e9, 1), "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def run_serve(ba, ctx, n_requests=240, rate_hz=40.0, seed=5, budget_frac=0.6, slots=2):
+    """Serve mode: a Poisson stream of LDE requests (8..256 columns, log_n uniform in 12..23,
+    blowup 2, 4 or 8, seeded synthetic inputs generated on the device) into a server whose
+    arena is 60% of HBM.  One client thread submits at the arrival times; a reaper thread
+    releases each result as soon as it is done.  Latency is submit -> completion."""
+    import threading
+    import numpy as np
+
+    free, total = ba.mem_get_info()
+    budget = min(int(total * budget_frac), int(free) - (8 << 30))
+    rng = np.random.default_rng(seed)
+    reqs = [(int(rng.integers(12, 24)), int(rng.integers(8, 257)), int(rng.choice([1, 2, 3])))
+            for _ in range(n_requests)]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, n_requests))
+    srv = ba.serve.Server(ctx, budget, slots)
+    outstanding, lock, finished = [], threading.Lock(), threading.Event()
+
+    def reaper():
+        while True:
+            with lock:
+                pending = list(outstanding)
+            for t in pending:
+                if srv.poll(t) is not None:
+                    srv.release(t)
+                    with lock:
+                        outstanding.remove(t)
+            with lock:
+                if finished.is_set() and not outstanding:
+                    return
+            time.sleep(0.0005)
+
+    th = threading.Thread(target=reaper)
+    th.start()
+    t0 = time.perf_counter()
+    for (log_n, cols, lb), at in zip(reqs, arrivals):
+        delay = t0 + at - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        t = srv.submit(log_n, cols, lb, seed=int(rng.integers(0, 2**63)))
+        with lock:
+            outstanding.append(t)
+    finished.set()
+    th.join()
+    st = srv.stats()
+    srv.destroy()
+    return {"workload": "Poisson stream of coset-LDE requests: 8..256 columns, log_n U{12..23}, blowup "
+                        "2/4/8, synthetic inputs filled on the device",
+            "requests": n_requests, "rate_hz": rate_hz, "slots": slots, "budget_bytes": st["budget_bytes"],
+            "peak_bytes": st["peak_bytes"], "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+            "mean_ms": round(st["mean_ms"], 3), "max_ms": round(st["max_ms"], 3),
+            "sustained_melem_s": round(st["output_elems"] / st["elapsed_s"] / 1e6, 1),
+            "offered_melem_s": round(sum(c << (l + b) for l, c, b in reqs) / arrivals[-1] / 1e6, 1),
+            "elapsed_s": round(st["elapsed_s"], 3), "batches": st["batches"],
+            "mean_batch_jobs": round(st["mean_batch_jobs"], 2), "failed": st["failed"]}
 
 
 def main():
@@ -286,6 +381,10 @@ def main():
     }
     if not args.no_uneven:
         line["uneven"] = run_uneven(ba, ctx, stream)
+    if not args.no_large:
+        line["large"] = run_large(ba, ctx, stream)
+    if not args.no_serve:
+        line["serve"] = run_serve(ba, ctx)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(log_n, lb)
     ba.check(lib.ba_graph_destroy(graph), "graph_destroy")

@@ -209,6 +209,60 @@ int ba_tuning_get(ba_context_t ctx, uint32_t log_n, int bitrev_inputs, ba_tuned_
 /* sets (plan != NULL, validated) or clears (plan == NULL) the tuned plan of a size class */
 int ba_tuning_set(ba_context_t ctx, uint32_t log_n, int bitrev_inputs, const ba_tuned_plan *plan);
 
+/* ---------------------------------------------------------------- serve mode
+ * No reference counterpart (shivini runs one prover's LDEs in program order,
+ * crates/shivini/src/data_structures/storage.rs:154-260): a request loop for coset-LDE jobs.
+ * Clients submit from any thread; the server's scheduler thread admits requests in arrival
+ * order while its device arena (one allocation of memory_budget_bytes) has room for the
+ * request's working buffer (2^log_n x num_cols elements) and outputs (2^(log_n+log_blowup) x
+ * num_cols), merges everything admitted since its last launch into one batch plan and enqueues
+ * it on a free slot stream (max_inflight_batches slots), so batches overlap on the device.
+ * Results are the same as ba_lde_batch on the same evaluations: column k, coset c of the
+ * output is at outputs + k * out_col_stride + c * 2^log_n (bit-reversed within the coset, the
+ * cosets in shivini's order).  Outputs stay resident until ba_server_release.  A request
+ * larger than the whole budget is refused by ba_server_submit with BA_ERROR_OUT_OF_MEMORY. */
+#define BA_SERVE_INPUT_HOST 0u      /* evals in host memory, copied in; valid until completion */
+#define BA_SERVE_INPUT_DEVICE 1u    /* evals in device memory, read (not modified) until completion */
+#define BA_SERVE_INPUT_SYNTHETIC 2u /* no evals: ba_fill_uniform(.., seed) on the device */
+typedef struct {
+  uint32_t log_n;
+  uint32_t num_cols;
+  uint32_t log_blowup;   /* 1.. ; log_n + log_blowup <= BA_OMEGA_LOG_ORDER */
+  uint32_t input_kind;   /* BA_SERVE_INPUT_* */
+  const uint64_t *evals; /* column-major, column k at evals + k * in_stride */
+  uint64_t in_stride;    /* 0 = 2^log_n */
+  uint64_t seed;         /* BA_SERVE_INPUT_SYNTHETIC */
+} ba_serve_request;
+typedef struct {
+  int32_t status;            /* 0, or the error that failed the request */
+  uint32_t reserved;
+  const uint64_t *outputs;   /* device memory owned by the server until released */
+  uint64_t out_col_stride;   /* 2^(log_n + log_blowup) */
+  double queue_ms;           /* submit -> admitted (memory granted) */
+  double latency_ms;         /* submit -> completion seen by the scheduler */
+  uint64_t batch;            /* the batch the request ran in */
+} ba_serve_result;
+typedef struct {
+  uint64_t completed, failed, batches, output_elems;
+  double p50_ms, p99_ms, mean_ms, max_ms; /* latency of completed requests */
+  double elapsed_s;                       /* first submit -> last completion */
+  double mean_batch_jobs;
+  uint64_t budget_bytes, peak_bytes, in_use_bytes;
+} ba_serve_stats;
+typedef struct ba_server *ba_server_t;
+int ba_server_create(ba_context_t ctx, size_t memory_budget_bytes, uint32_t max_inflight_batches,
+                     ba_server_t *server);
+int ba_server_submit(ba_server_t server, const ba_serve_request *request, uint64_t *ticket);
+/* blocks until the request completed or failed */
+int ba_server_wait(ba_server_t server, uint64_t ticket, ba_serve_result *result);
+/* *done = 1 (and *result filled, if given) once the request completed or failed */
+int ba_server_poll(ba_server_t server, uint64_t ticket, int *done, ba_serve_result *result);
+/* returns a finished request's outputs to the arena; the ticket is then forgotten */
+int ba_server_release(ba_server_t server, uint64_t ticket);
+int ba_server_stats(ba_server_t server, ba_serve_stats *stats);
+/* completes every submitted request, then frees the arena and the slots */
+int ba_server_destroy(ba_server_t server);
+
 /* ---------------------------------------------------------------- launch tracing
  * No reference counterpart beyond its nvtx ranges (crates/shivini/Cargo.toml:31): after
  * ba_profile_begin(events, capacity) on a host thread, each kernel the library launches from

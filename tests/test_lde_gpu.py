@@ -123,6 +123,42 @@ def test_lde_headline_size_samples(ba):
             t.free()
 
 
+def test_lde_large_trace_samples(ba):
+    """256 columns of 2^22 at blowup 4 (2^32 output elements, 34.4 GB): two columns checked
+    against the oracle, eight columns of every coset through the interpolation round trip."""
+    log_n, cols, lb = 22, 256, 2
+    n, B = 1 << log_n, 1 << lb
+    sample = [0, 1, 77, 128, 200, 254, 255, 31]
+    d_ev = ba.DeviceAllocation(n * cols)
+    d_out = ba.DeviceAllocation(B * n * cols)
+    d_mono = ba.DeviceAllocation(n * cols)
+    d_tmp = ba.DeviceAllocation(n * len(sample))
+    try:
+        ba.runtime.fill_uniform(d_ev, n, cols, 4242)
+        log("large trace LDE 256 x 2^22 blowup 4")
+        ba.lde.lde_batch(ba._test_ctx, d_ev, log_n, cols, lb, d_out, monomials=d_mono)
+        ba.synchronize()
+        ev = {k: d_ev.sub(k * n, n).to_host() for k in sample}
+        for k in (1, 254):
+            want = gl_oracle.lde(ev[k], log_n, 1, lb, nthreads=8)
+            assert np.array_equal(d_out.sub(k * B * n, B * n).to_host(), want), k
+        perm = bitrev_perm(log_n)
+        for c in range(B):
+            coset_idx = int(format(c, f"0{lb}b")[::-1], 2)
+            for i, k in enumerate(sample):
+                d_tmp.sub(i * n, n).copy_from_device(d_out.sub(k * B * n + c * n, n))
+            m = len(sample)
+            ba.ntt.batch_ntt_internal(d_tmp.ptr, d_tmp.ptr, log_n, m, n, n, True, True, lb, coset_idx)
+            ba.ntt.batch_ntt_internal(d_tmp.ptr, d_tmp.ptr, log_n, m, n, n, False, False, 0, 0)
+            ba.synchronize()
+            back = d_tmp.to_host()
+            for i, k in enumerate(sample):
+                assert np.array_equal(back[i * n:(i + 1) * n][perm], ev[k]), (c, k)
+    finally:
+        for t in (d_ev, d_out, d_mono, d_tmp):
+            t.free()
+
+
 def test_lde_rejects_bad_arguments(ba):
     d = ba.DeviceAllocation(1 << 10)
     o = ba.DeviceAllocation(1 << 12)

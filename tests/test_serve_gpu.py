@@ -1,0 +1,144 @@
+"""GPU tests of serve mode (ba_server_*): every request's result is the same when it runs alone
+and when it runs in a mix of concurrently submitted requests, and matches the oracle; admission
+keeps the arena within its budget; invalid requests are refused."""
+import threading
+
+import numpy as np
+import pytest
+
+import gl_oracle
+from gpu_util import column_seed, log, raw_u64
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ba():
+    import boojum_amd
+    assert boojum_amd.device_count() >= 1, "no GPU visible"
+    ctx = boojum_amd.Context.create(12, 12)
+    boojum_amd._test_ctx = ctx
+    yield boojum_amd
+    ctx.destroy()
+
+
+def make_requests(ba, rng, count):
+    reqs = []
+    for i in range(count):
+        log_n = int(rng.choice([3, 5, 10, 12, 13, 16]))
+        cols = int(rng.integers(1, 6))
+        lb = int(rng.choice([1, 2, 3]))
+        n = 1 << log_n
+        kind = ("host", "device", "synthetic")[i % 3]
+        r = dict(log_n=log_n, cols=cols, lb=lb, kind=kind, seed=900 + i)
+        if kind == "synthetic":
+            r["x"] = np.concatenate([gl_oracle.fill_uniform(n, column_seed(r["seed"], k)) for k in range(cols)])
+        else:
+            stride = n + (i % 2) * 3
+            r["stride"] = stride
+            r["x"] = raw_u64(rng, cols * stride)
+            if kind == "device":
+                r["dev"] = ba.DeviceAllocation.from_host(r["x"])
+        reqs.append(r)
+    return reqs
+
+
+def submit(srv, r):
+    if r["kind"] == "synthetic":
+        return srv.submit(r["log_n"], r["cols"], r["lb"], seed=r["seed"])
+    src = r["x"] if r["kind"] == "host" else r["dev"]
+    return srv.submit(r["log_n"], r["cols"], r["lb"], evals=src, in_stride=r["stride"])
+
+
+def fetch(srv, r, res):
+    assert res["status"] == 0, res
+    out = srv.output(res, r["log_n"], r["cols"], r["lb"]).to_host()
+    srv.release(res["ticket"])
+    return out
+
+
+def test_alone_equals_mixed_and_oracle(ba):
+    rng = np.random.default_rng(11)
+    reqs = make_requests(ba, rng, 18)
+    srv = ba.serve.Server(ba._test_ctx, 1 << 30, max_inflight_batches=2)
+    try:
+        alone = []
+        for r in reqs:  # each request alone: submit, wait, fetch
+            alone.append(fetch(srv, r, srv.wait(submit(srv, r))))
+        # the same requests submitted together from three client threads
+        tickets = [None] * len(reqs)
+
+        def client(idx):
+            for i in idx:
+                tickets[i] = submit(srv, reqs[i])
+
+        th = [threading.Thread(target=client, args=(list(range(k, len(reqs), 3)),)) for k in range(3)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        results = [srv.wait(t) for t in tickets]
+        batches = {res["batch"] for res in results}
+        log(f"mixed: {len(reqs)} requests in {len(batches)} batches; stats {srv.stats()}")
+        for r, res, a in zip(reqs, results, alone):
+            got = fetch(srv, r, res)
+            assert np.array_equal(got, a), (r["log_n"], r["cols"], r["lb"], r["kind"])
+            stride = r.get("stride", 1 << r["log_n"])
+            want = gl_oracle.lde(r["x"], r["log_n"], r["cols"], r["lb"], in_stride=stride, nthreads=8)
+            assert np.array_equal(got, want), (r["log_n"], r["cols"], r["lb"], r["kind"])
+            if r["kind"] == "device":  # device inputs are read, not modified
+                assert np.array_equal(r["dev"].to_host(), r["x"])
+        st = srv.stats()
+        assert st["completed"] == 2 * len(reqs) and st["failed"] == 0
+        assert st["in_use_bytes"] == 0 and st["peak_bytes"] <= st["budget_bytes"]
+        assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+        assert st["batches"] < 2 * len(reqs)  # the concurrent submissions were batched together
+    finally:
+        srv.destroy()
+        for r in reqs:
+            if "dev" in r:
+                r["dev"].free()
+
+
+def test_admission_within_budget(ba):
+    """A budget that holds about two requests at a time: every request completes, the arena
+    never exceeds the budget, and a request larger than the budget is refused at submit."""
+    log_n, cols, lb = 16, 4, 2
+    per_req = (cols << log_n) * 8 * (1 + (1 << lb))
+    budget = 2 * per_req + 4096
+    srv = ba.serve.Server(ba._test_ctx, budget, max_inflight_batches=2)
+    try:
+        tickets = [srv.submit(log_n, cols, lb, seed=50 + i) for i in range(10)]
+        done = {}
+        for t in tickets:  # wait in order and release, which lets the next requests in
+            res = srv.wait(t)
+            assert res["status"] == 0
+            done[t] = srv.output(res, log_n, cols, lb).to_host()
+            srv.release(t)
+        st = srv.stats()
+        log(f"budget {budget}: {st}")
+        assert st["completed"] == 10 and st["peak_bytes"] <= budget
+        # request 3 against the oracle
+        x = np.concatenate([gl_oracle.fill_uniform(1 << log_n, column_seed(53, k)) for k in range(cols)])
+        assert np.array_equal(done[tickets[3]], gl_oracle.lde(x, log_n, cols, lb, nthreads=8))
+        with pytest.raises(ba.BoojumAmdError) as e:
+            srv.submit(18, 4, 2, seed=1)
+        assert e.value.code == 2  # BA_ERROR_OUT_OF_MEMORY
+    finally:
+        srv.destroy()
+
+
+def test_bad_requests(ba):
+    srv = ba.serve.Server(ba._test_ctx, 1 << 24)
+    try:
+        for log_n, cols, lb in [(0, 1, 1), (10, 0, 1), (10, 1, 0), (29, 1, 1)]:
+            with pytest.raises(ba.BoojumAmdError):
+                srv.submit(log_n, cols, lb, seed=1)
+        with pytest.raises(ba.BoojumAmdError):
+            srv.release(12345)
+        with pytest.raises(ba.BoojumAmdError):
+            srv.poll(12345)
+    finally:
+        srv.destroy()
+    with pytest.raises(ba.BoojumAmdError):
+        ba.serve.Server(ba._test_ctx, 1 << 24, max_inflight_batches=0)

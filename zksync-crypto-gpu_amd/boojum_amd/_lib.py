@@ -22,6 +22,27 @@ class TunedPlan(ctypes.Structure):
     _fields_ = [("num_passes", _u32), ("bits", _u32 * 8), ("reps", _u32 * 8)]
 
 
+class ServeRequest(ctypes.Structure):
+    """ba_serve_request (include/boojum_amd.h)."""
+    _fields_ = [("log_n", _u32), ("num_cols", _u32), ("log_blowup", _u32), ("input_kind", _u32),
+                ("evals", _vp), ("in_stride", _u64), ("seed", _u64)]
+
+
+class ServeResult(ctypes.Structure):
+    """ba_serve_result (include/boojum_amd.h)."""
+    _fields_ = [("status", _i32), ("reserved", _u32), ("outputs", _vp), ("out_col_stride", _u64),
+                ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double), ("batch", _u64)]
+
+
+class ServeStats(ctypes.Structure):
+    """ba_serve_stats (include/boojum_amd.h)."""
+    _fields_ = [("completed", _u64), ("failed", _u64), ("batches", _u64), ("output_elems", _u64),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
+                ("max_ms", ctypes.c_double), ("elapsed_s", ctypes.c_double),
+                ("mean_batch_jobs", ctypes.c_double), ("budget_bytes", _u64), ("peak_bytes", _u64),
+                ("in_use_bytes", _u64)]
+
+
 # name -> argtypes (restype int unless listed in _RESTYPES)
 SIGNATURES = {
     "ba_get_device_count": [ctypes.POINTER(_i32)],
@@ -68,6 +89,13 @@ SIGNATURES = {
     "ba_batch_plan_stats": [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32), ctypes.POINTER(_u64)],
     "ba_batch_plan_destroy": [_vp],
     "ba_autotune": [_vp, _u32, _u32, _u64, _vp],
+    "ba_server_create": [_vp, _sz, _u32, ctypes.POINTER(_vp)],
+    "ba_server_submit": [_vp, ctypes.POINTER(ServeRequest), ctypes.POINTER(_u64)],
+    "ba_server_wait": [_vp, _u64, ctypes.POINTER(ServeResult)],
+    "ba_server_poll": [_vp, _u64, ctypes.POINTER(_i32), ctypes.POINTER(ServeResult)],
+    "ba_server_release": [_vp, _u64],
+    "ba_server_stats": [_vp, ctypes.POINTER(ServeStats)],
+    "ba_server_destroy": [_vp],
     "ba_tuning_get": [_vp, _u32, _i32, ctypes.POINTER(TunedPlan)],
     "ba_tuning_set": [_vp, _u32, _i32, ctypes.POINTER(TunedPlan)],
     "ba_profile_begin": [ctypes.POINTER(_vp), _u32],

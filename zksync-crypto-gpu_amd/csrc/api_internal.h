@@ -52,6 +52,7 @@ bool ba_plan_from_bits(uint32_t log_n, bool n2b, const uint32_t *bits, int num_p
 NttPlan ba_plan_for(const ba_context *ctx, uint32_t log_n, bool n2b);
 
 // One pass launch of a batch NTT: the kernel pair of its shape, its arguments and grid.
+#include <utility>
 #include <vector>
 #include "ntt_inst.h"
 struct BaPassLaunch {
@@ -78,6 +79,33 @@ struct BaLeveledLaunch {
   uint32_t level;
   BaPassLaunch launch;
 };
+// A batch plan as data: grouped launches (one per dependency level and pass shape) and the
+// tables they read, laid out for one device allocation of `bytes` bytes.
+struct BaGroup {
+  BaGroupKernel kernel;
+  const char *name;
+  uint32_t level;
+  uint32_t njobs;
+  uint64_t workgroups;
+  size_t args_off;    // index into the PassArgs table
+  size_t prefix_off;  // index into the prefix table
+};
+struct BaPlanImage {
+  std::vector<BaGroup> groups;  // ordered by level, largest first within a level
+  uint32_t levels = 0;
+  uint64_t workgroups = 0;
+  std::vector<gl::ntt::PassArgs> args;
+  std::vector<std::pair<size_t, size_t>> lists;  // (in, out) offsets into `table` per PassArgs
+  std::vector<const void *> table;               // column pointers
+  std::vector<uint32_t> prefix;                  // first workgroup of each PassArgs in its group
+  size_t table_at = 0, prefix_at = 0, bytes = 0;
+};
+int ba_plan_image_build(const ba_context *ctx, const ba_job *jobs, uint32_t num_jobs, BaPlanImage &img);
+// Serialises the image into `host` (img.bytes bytes) for a copy to device address `device`.
+void ba_plan_image_write(const BaPlanImage &img, void *host, void *device);
+// Enqueues one group of an image uploaded at `device`.
+hipError_t ba_plan_launch_group(const BaGroup &g, const void *device, const BaPlanImage &img, hipStream_t s);
+
 int ba_validate_lde(uint64_t *evals, uint64_t in_stride, uint64_t *monomials, uint64_t *out,
                     uint64_t out_col_stride, uint64_t out_coset_stride, uint32_t log_n,
                     uint32_t num_cols, uint32_t log_blowup);

@@ -14,6 +14,7 @@
 // stream, so the whole batch is (#levels x #shapes per level) launches, capturable as one
 // hipGraph, with no synchronisation between workgroups.
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -23,22 +24,9 @@
 static_assert(sizeof(ba_job) == 80, "ba_job layout is part of the ABI");
 
 struct ba_batch_plan {
-  struct Group {
-    BaGroupKernel kernel;
-    const char *name;
-    uint32_t level;
-    uint32_t njobs;
-    uint64_t workgroups;
-    size_t args_off;    // index into the PassArgs table
-    size_t prefix_off;  // index into the prefix table
-  };
   int device = 0;
-  void *mem = nullptr;  // PassArgs table, column pointer lists, prefix table
-  gl::ntt::PassArgs *d_args = nullptr;
-  uint32_t *d_prefix = nullptr;
-  std::vector<Group> groups;  // ordered by level, largest group first within a level
-  uint32_t levels = 0;
-  uint64_t workgroups = 0;
+  void *mem = nullptr;  // the uploaded image: PassArgs table, column pointer lists, prefix table
+  BaPlanImage image;    // groups ordered by level, largest group first within a level
   // groups of one level are independent: they fork onto auxiliary streams (all at the default
   // priority) and join back, so a level runs as concurrent branches, in a graph too
   static constexpr int kAux = 4;
@@ -120,11 +108,8 @@ int add_transform(const ba_context *ctx, uint32_t log_n, bool bitrev, bool inver
 
 }  // namespace
 
-extern "C" int ba_batch_plan_create(ba_context_t ctx, const ba_job *jobs, uint32_t num_jobs,
-                                    ba_batch_plan_t *plan) {
-  if (!plan) return BA_ERROR_INVALID_VALUE;
-  *plan = nullptr;
-  if (!ctx) return BA_ERROR_NOT_INITIALIZED;
+int ba_plan_image_build(const ba_context *ctx, const ba_job *jobs, uint32_t num_jobs, BaPlanImage &img) {
+  img = BaPlanImage{};
   if (num_jobs && !jobs) return BA_ERROR_INVALID_VALUE;
   // 1. columns of all jobs, merged by signature
   std::map<Signature, Columns> merged;
@@ -155,142 +140,144 @@ extern "C" int ba_batch_plan_create(ba_context_t ctx, const ba_job *jobs, uint32
     }
   }
   // 2. pass launches of every merged transform
-  std::vector<const void *> table;
   std::vector<PendingLaunch> pending;
   for (const auto &kv : merged) {
     const Signature &sg = kv.first;
     const Columns &c = kv.second;
     int rc;
     if (sg.kind == BA_JOB_NTT) {
-      rc = add_transform(ctx, sg.log_n, sg.bitrev, sg.inverse, sg.log_ext, sg.coset, false, c.in, c.out, 0, table,
+      rc = add_transform(ctx, sg.log_n, sg.bitrev, sg.inverse, sg.log_ext, sg.coset, false, c.in, c.out, 0, img.table,
                          pending, nullptr);
     } else {
       // n2b inverse NTT to bit-reversed monomials, then every coset's forward NTT gathers them
       uint32_t depth = 0;
-      rc = add_transform(ctx, sg.log_n, false, true, 0, 0, false, c.in, c.mono, 0, table, pending, &depth);
+      rc = add_transform(ctx, sg.log_n, false, true, 0, 0, false, c.in, c.mono, 0, img.table, pending, &depth);
       for (uint32_t cs = 0; rc == BA_SUCCESS && cs < c.coset.size(); cs++)
         rc = add_transform(ctx, sg.log_n, false, false, sg.log_blowup, bitrev_bits(cs, sg.log_blowup), true,
-                           c.mono, c.coset[cs], depth, table, pending, nullptr);
+                           c.mono, c.coset[cs], depth, img.table, pending, nullptr);
     }
     if (rc) return rc;
   }
   // 3. group by (level, shape); std::map keeps levels ascending
   std::map<std::pair<uint32_t, void *>, std::vector<const PendingLaunch *>> by_group;
   for (const PendingLaunch &l : pending) by_group[{l.level, (void *)l.launch.kernels.group}].push_back(&l);
-
-  ba_batch_plan *p = new ba_batch_plan();
-  (void)hipGetDevice(&p->device);
-  std::vector<gl::ntt::PassArgs> h_args;
-  std::vector<std::pair<size_t, size_t>> h_lists;  // (in, out) pointer-table offsets per PassArgs
-  std::vector<uint32_t> h_prefix;
   for (auto &kv : by_group) {
     // split a group whose workgroup count would overflow the 32-bit prefix table or grid
     size_t i = 0;
     const auto &ls = kv.second;
     while (i < ls.size()) {
-      ba_batch_plan::Group g{};
+      BaGroup g{};
       g.kernel = ls[i]->launch.kernels.group;
       g.name = ls[i]->launch.name;
       g.level = kv.first.first;
-      g.args_off = h_args.size();
-      g.prefix_off = h_prefix.size();
+      g.args_off = img.args.size();
+      g.prefix_off = img.prefix.size();
       uint64_t wg = 0;
       while (i < ls.size() && wg + ls[i]->launch.grid <= 0x7FFFFFFFull) {
-        h_prefix.push_back((uint32_t)wg);
-        h_args.push_back(ls[i]->launch.args);
-        h_lists.push_back({ls[i]->in_list, ls[i]->out_list});
+        img.prefix.push_back((uint32_t)wg);
+        img.args.push_back(ls[i]->launch.args);
+        img.lists.push_back({ls[i]->in_list, ls[i]->out_list});
         wg += ls[i]->launch.grid;
         i++;
       }
-      if (h_args.size() == g.args_off) {  // a single launch larger than a grid: cannot happen
-        delete p;
-        return BA_ERROR_INVALID_VALUE;
-      }
-      g.njobs = (uint32_t)(h_args.size() - g.args_off);
+      if (img.args.size() == g.args_off) return BA_ERROR_INVALID_VALUE;  // one launch larger than a grid
+      g.njobs = (uint32_t)(img.args.size() - g.args_off);
       g.workgroups = wg;
-      p->workgroups += wg;
-      p->levels = std::max(p->levels, g.level + 1);
-      p->groups.push_back(g);
+      img.workgroups += wg;
+      img.levels = std::max(img.levels, g.level + 1);
+      img.groups.push_back(g);
     }
   }
-  std::stable_sort(p->groups.begin(), p->groups.end(), [](const ba_batch_plan::Group &a, const ba_batch_plan::Group &b) {
+  std::stable_sort(img.groups.begin(), img.groups.end(), [](const BaGroup &a, const BaGroup &b) {
     return a.level != b.level ? a.level < b.level : a.workgroups > b.workgroups;
   });
-  {
-    hipError_t err = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
-    for (int i = 0; err == hipSuccess && i < ba_batch_plan::kAux; i++)
-      err = hipStreamCreateWithFlags(&p->aux[i], hipStreamNonBlocking);
-    p->done.resize(p->groups.size(), nullptr);
-    for (size_t i = 0; err == hipSuccess && i < p->done.size(); i++)
-      err = hipEventCreateWithFlags(&p->done[i], hipEventDisableTiming);
-    if (err != hipSuccess) {
-      delete p;
-      return (int)err;
+  // device layout: PassArgs table, pointer table, prefix table, each 256-byte aligned
+  const size_t args_bytes = img.args.size() * sizeof(gl::ntt::PassArgs);
+  img.table_at = (args_bytes + 255) & ~(size_t)255;
+  img.prefix_at = (img.table_at + img.table.size() * sizeof(void *) + 255) & ~(size_t)255;
+  img.bytes = img.args.empty() ? 0 : img.prefix_at + img.prefix.size() * sizeof(uint32_t);
+  return BA_SUCCESS;
+}
+
+void ba_plan_image_write(const BaPlanImage &img, void *host, void *device) {
+  char *h = (char *)host;
+  const void **d_table = (const void **)((char *)device + img.table_at);
+  gl::ntt::PassArgs *ha = (gl::ntt::PassArgs *)h;
+  for (size_t k = 0; k < img.args.size(); k++) {
+    ha[k] = img.args[k];
+    ha[k].in_cols = (const uint64_t *const *)(d_table + img.lists[k].first);
+    ha[k].out_cols = (uint64_t *const *)(d_table + img.lists[k].second);
+  }
+  if (!img.table.empty()) memcpy(h + img.table_at, img.table.data(), img.table.size() * sizeof(void *));
+  if (!img.prefix.empty()) memcpy(h + img.prefix_at, img.prefix.data(), img.prefix.size() * sizeof(uint32_t));
+}
+
+hipError_t ba_plan_launch_group(const BaGroup &g, const void *device, const BaPlanImage &img, hipStream_t s) {
+  const gl::ntt::PassArgs *d_args = (const gl::ntt::PassArgs *)device + g.args_off;
+  const uint32_t *d_prefix = (const uint32_t *)((const char *)device + img.prefix_at) + g.prefix_off;
+  ba_trace_before(s, g.name);
+  hipLaunchKernelGGL(g.kernel, dim3((uint32_t)g.workgroups), dim3(gl::ntt::THREADS), 0, s, d_args, d_prefix,
+                     g.njobs);
+  ba_trace_after(s);
+  return hipGetLastError();
+}
+
+extern "C" int ba_batch_plan_create(ba_context_t ctx, const ba_job *jobs, uint32_t num_jobs,
+                                    ba_batch_plan_t *plan) {
+  if (!plan) return BA_ERROR_INVALID_VALUE;
+  *plan = nullptr;
+  if (!ctx) return BA_ERROR_NOT_INITIALIZED;
+  ba_batch_plan *p = new ba_batch_plan();
+  (void)hipGetDevice(&p->device);
+  int rc = ba_plan_image_build(ctx, jobs, num_jobs, p->image);
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  hipError_t err = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
+  for (int i = 0; err == hipSuccess && i < ba_batch_plan::kAux; i++)
+    err = hipStreamCreateWithFlags(&p->aux[i], hipStreamNonBlocking);
+  p->done.resize(p->image.groups.size(), nullptr);
+  for (size_t i = 0; err == hipSuccess && i < p->done.size(); i++)
+    err = hipEventCreateWithFlags(&p->done[i], hipEventDisableTiming);
+  if (err == hipSuccess && p->image.bytes) {
+    err = hipMalloc(&p->mem, p->image.bytes);
+    if (err != hipSuccess) p->mem = nullptr;
+    if (err == hipSuccess) {
+      std::vector<char> host(p->image.bytes);
+      ba_plan_image_write(p->image, host.data(), p->mem);
+      err = hipMemcpy(p->mem, host.data(), host.size(), hipMemcpyHostToDevice);
     }
   }
-  if (!h_args.empty()) {
-    const size_t args_bytes = h_args.size() * sizeof(gl::ntt::PassArgs);
-    const size_t table_bytes = table.size() * sizeof(void *);
-    const size_t prefix_bytes = h_prefix.size() * sizeof(uint32_t);
-    const size_t table_at = (args_bytes + 255) & ~(size_t)255;
-    const size_t prefix_at = (table_at + table_bytes + 255) & ~(size_t)255;
-    hipError_t err = hipMalloc(&p->mem, prefix_at + prefix_bytes);
-    if (err != hipSuccess) {
-      p->mem = nullptr;
-      delete p;
-      return (int)err;
-    }
-    p->d_args = (gl::ntt::PassArgs *)p->mem;
-    const void **d_table = (const void **)((char *)p->mem + table_at);
-    p->d_prefix = (uint32_t *)((char *)p->mem + prefix_at);
-    for (size_t k = 0; k < h_args.size(); k++) {
-      h_args[k].in_cols = (const uint64_t *const *)(d_table + h_lists[k].first);
-      h_args[k].out_cols = (uint64_t *const *)(d_table + h_lists[k].second);
-    }
-    err = hipMemcpy(p->d_args, h_args.data(), args_bytes, hipMemcpyHostToDevice);
-    if (err == hipSuccess) err = hipMemcpy(d_table, table.data(), table_bytes, hipMemcpyHostToDevice);
-    if (err == hipSuccess) err = hipMemcpy(p->d_prefix, h_prefix.data(), prefix_bytes, hipMemcpyHostToDevice);
-    if (err != hipSuccess) {
-      delete p;
-      return (int)err;
-    }
+  if (err != hipSuccess) {
+    delete p;
+    return (int)err;
   }
   *plan = p;
   return BA_SUCCESS;
 }
 
-namespace {
-
-hipError_t launch_group(ba_batch_plan *plan, const ba_batch_plan::Group &g, hipStream_t s) {
-  ba_trace_before(s, g.name);
-  hipLaunchKernelGGL(g.kernel, dim3((uint32_t)g.workgroups), dim3(gl::ntt::THREADS), 0, s,
-                     plan->d_args + g.args_off, plan->d_prefix + g.prefix_off, g.njobs);
-  ba_trace_after(s);
-  return hipGetLastError();
-}
-
-}  // namespace
-
 extern "C" int ba_batch_plan_launch(ba_batch_plan_t plan, ba_stream_t stream) {
   if (!plan) return BA_ERROR_INVALID_VALUE;
   hipStream_t s = (hipStream_t)stream;
+  const std::vector<BaGroup> &groups = plan->image.groups;
   size_t i = 0;
-  while (i < plan->groups.size()) {
+  while (i < groups.size()) {
     size_t e = i;
-    while (e < plan->groups.size() && plan->groups[e].level == plan->groups[i].level) e++;
+    while (e < groups.size() && groups[e].level == groups[i].level) e++;
     hipError_t err = hipSuccess;
     if (e - i == 1) {
-      err = launch_group(plan, plan->groups[i], s);
+      err = ba_plan_launch_group(groups[i], plan->mem, plan->image, s);
     } else {
       // the largest group stays on the caller's stream, the others fork and join
       err = hipEventRecord(plan->fork, s);
       for (size_t k = i + 1; err == hipSuccess && k < e; k++) {
         hipStream_t a = plan->aux[(k - i - 1) % ba_batch_plan::kAux];
         err = hipStreamWaitEvent(a, plan->fork, 0);
-        if (err == hipSuccess) err = launch_group(plan, plan->groups[k], a);
+        if (err == hipSuccess) err = ba_plan_launch_group(groups[k], plan->mem, plan->image, a);
         if (err == hipSuccess) err = hipEventRecord(plan->done[k], a);
       }
-      if (err == hipSuccess) err = launch_group(plan, plan->groups[i], s);
+      if (err == hipSuccess) err = ba_plan_launch_group(groups[i], plan->mem, plan->image, s);
       for (size_t k = i + 1; err == hipSuccess && k < e; k++) err = hipStreamWaitEvent(s, plan->done[k], 0);
     }
     if (err != hipSuccess) return (int)err;
@@ -302,9 +289,9 @@ extern "C" int ba_batch_plan_launch(ba_batch_plan_t plan, ba_stream_t stream) {
 extern "C" int ba_batch_plan_stats(ba_batch_plan_t plan, uint32_t *launches, uint32_t *levels,
                                    uint64_t *workgroups) {
   if (!plan) return BA_ERROR_INVALID_VALUE;
-  if (launches) *launches = (uint32_t)plan->groups.size();
-  if (levels) *levels = plan->levels;
-  if (workgroups) *workgroups = plan->workgroups;
+  if (launches) *launches = (uint32_t)plan->image.groups.size();
+  if (levels) *levels = plan->image.levels;
+  if (workgroups) *workgroups = plan->image.workgroups;
   return BA_SUCCESS;
 }
 
