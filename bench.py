@@ -40,9 +40,13 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-per-call", action="store_true", help="skip the per-column-call leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch leg")
-    ap.add_argument("--no-tune", action="store_true", help="use the built-in pass plans, no autotuning")
+    ap.add_argument("--no-tune", action="store_true", help="uneven leg: built-in pass plans, no autotuning")
+    ap.add_argument("--tune-headline", action="store_true",
+                    help="autotune the headline size class too (default: built-in plan, which the tuner "
+                         "keeps for 2^20)")
     ap.add_argument("--no-large", action="store_true", help="skip the 256 x 2^22 blowup-4 leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serve-mode leg")
+    ap.add_argument("--no-single", action="store_true", help="skip the single 2^16 NTT leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a 2^27-element copy kernel (canonicalise, 8 B read + 8 B written "
                          "per element) that scales the FETCH_SIZE/WRITE_SIZE counters")
@@ -62,6 +66,12 @@ def cpu_baseline(log_n, log_blowup):
     gl_oracle.lde(ev, log_n, cols, log_blowup, nthreads=threads)
     dt = time.perf_counter() - t0
     out_elems = cols * (n << log_blowup)
+    # the single-column 2^16 forward NTT on one core
+    x16 = gl_oracle.fill_uniform(1 << 16, 0)
+    t1 = time.perf_counter()
+    for _ in range(20):
+        gl_oracle.batch_ntt(x16, 16, 1, 1 << 16, 1 << 16, False, False, nthreads=1)
+    single_us = (time.perf_counter() - t1) / 20 * 1e6
     return {
         "value": round(out_elems / dt / 1e6, 3),
         "unit": "Melem/s",
@@ -69,7 +79,34 @@ def cpu_baseline(log_n, log_blowup):
         "kind": "port",
         "sample": f"{cols} columns of the same 2^{log_n} blowup-{1 << log_blowup} LDE, one per thread, "
                   f"{dt:.2f} s wall",
+        "single_ntt_2p16_us_1core": round(single_us, 1),
     }
+
+
+def run_single(ba, ctx, stream, log_n=16, calls=200):
+    """One column of 2^16, natural -> bit-reversed forward NTT, one library call per transform
+    (the reference's CPU-runnable case), inputs resident in HBM."""
+    n = 1 << log_n
+    d_in, d_out = ba.DeviceAllocation(n), ba.DeviceAllocation(n)
+    try:
+        ba.runtime.fill_uniform(d_in, n, 1, 0)
+        for _ in range(10):
+            ba.ntt.batch_ntt_ctx(ctx, d_in.ptr, d_out.ptr, log_n, 1, n, n, False, False, stream=stream)
+        stream.synchronize()
+        e0, e1 = ba.CudaEvent(), ba.CudaEvent()
+        e0.record(stream)
+        for _ in range(calls):
+            ba.ntt.batch_ntt_ctx(ctx, d_in.ptr, d_out.ptr, log_n, 1, n, n, False, False, stream=stream)
+        e1.record(stream)
+        e1.synchronize()
+        us = e0.elapsed_ms(e1) / calls * 1e3
+        e0.destroy()
+        e1.destroy()
+    finally:
+        d_in.free()
+        d_out.free()
+    return {"workload": f"1 column x 2^{log_n} forward NTT, natural in, bit-reversed out, one call each",
+            "us_per_call": round(us, 2), "melem_s": round(n / us, 1)}
 
 
 def run_uneven(ba, ctx, stream, seed=3, jobs_n=512, reps=5):
@@ -253,12 +290,10 @@ def main():
     ctx = ba.Context.create(12, 12)
     log_n, cols, lb = args.log_n, args.cols, args.log_blowup
     tuning = None
-    if not args.no_tune:
-        # pass split and columns per workgroup for every size class the legs below run
-        lo, hi = min(12, log_n), max(22, log_n)
+    if args.tune_headline:
         t0 = time.perf_counter()
-        ba.tune.autotune(ctx, lo, hi, elems_per_trial=1 << 26)
-        tuning = {"log_n_range": [lo, hi], "seconds": round(time.perf_counter() - t0, 2),
+        ba.tune.autotune(ctx, log_n, log_n, elems_per_trial=n * cols)
+        tuning = {"log_n_range": [log_n, log_n], "seconds": round(time.perf_counter() - t0, 2),
                   f"plan_{log_n}_natural": ba.tune.get_plan(ctx, log_n, False),
                   f"plan_{log_n}_bitrev": ba.tune.get_plan(ctx, log_n, True)}
     n, B = 1 << log_n, 1 << lb
@@ -380,7 +415,19 @@ def main():
         "wall_s": round(wall, 3),
     }
     if not args.no_uneven:
-        line["uneven"] = run_uneven(ba, ctx, stream)
+        # its own context, autotuned over the batch's size classes (tuning launches run before
+        # and apart from every timed or traced region)
+        uctx = ba.Context.create(12, 12)
+        utune = None
+        if not args.no_tune:
+            t0 = time.perf_counter()
+            ba.tune.autotune(uctx, 12, 22, elems_per_trial=1 << 26)
+            utune = {"log_n_range": [12, 22], "seconds": round(time.perf_counter() - t0, 2)}
+        line["uneven"] = run_uneven(ba, uctx, stream)
+        line["uneven"]["tuning"] = utune
+        uctx.destroy()
+    if not args.no_single:
+        line["single"] = run_single(ba, ctx, stream)
     if not args.no_large:
         line["large"] = run_large(ba, ctx, stream)
     if not args.no_serve:

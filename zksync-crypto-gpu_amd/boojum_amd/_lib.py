@@ -3,7 +3,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-library_path = os.path.join(os.path.dirname(_HERE), "libboojum_amd.so")
+# BOOJUM_AMD_LIBRARY overrides the in-tree build (e.g. to compare two builds of the library)
+library_path = os.environ.get("BOOJUM_AMD_LIBRARY") or os.path.join(os.path.dirname(_HERE), "libboojum_amd.so")
 
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 _vp = ctypes.c_void_p

@@ -234,11 +234,14 @@ struct Shape {
 
 // W for a round whose active bits are [P, P + K) of an L-bit index; gidx = any in-column
 // index of the thread's elements in that round.
-__device__ __forceinline__ u64 round_twiddle(const PassArgs &a, u32 gidx, u32 P, u32 K, bool inverse) {
+__device__ __forceinline__ u32 round_exp(const PassArgs &a, u32 gidx, u32 P, u32 K) {
   const u32 L = a.log_n;
   const u32 hi_bits = L - P - K;
-  if (!hi_bits) return 1;
-  const u32 e = bitrev32(gidx >> (P + K), hi_bits) << (OMEGA_LOG_ORDER - (L - P));
+  if (!hi_bits) return 0;
+  return bitrev32(gidx >> (P + K), hi_bits) << (OMEGA_LOG_ORDER - (L - P));
+}
+__device__ __forceinline__ u64 round_twiddle(const PassArgs &a, u32 gidx, u32 P, u32 K, bool inverse) {
+  const u32 e = round_exp(a, gidx, P, K);
   return e ? pow_w(a.tab, e, inverse) : 1;
 }
 
@@ -413,11 +416,28 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
   constexpr bool whole = WHOLE;
   const u32 tile_in_col = (u32)((tile0 << (LOG_C + S)) & ((1ull << L) - 1));  // first element
 
+  // Twiddles.  A round on the window starting at bit WB >= 4 has at most 2^(8-WB) <= 16
+  // distinct W per tile (the tile's 2^(12-S) chunks times the 2^(S-4-WB) values of q >> WB),
+  // so the workgroup keeps their powers in LDS, one entry per thread to fill; a round on a
+  // lower window (every thread its own W) keeps its powers in registers, across repeats.
+  __shared__ u64 ltw[R][256];
   u64 tw[R][VALS];
   static_for<R>([&](auto r_) {
     constexpr int r = decltype(r_)::value;
-    const u64 w = round_twiddle(a, hbase + window_a<Sh::WB(r)>(q, 0), Sh::AB(r), Sh::K(r), INV);
-    twiddle_powers<Sh::K(r)>(tw[r], w);
+    constexpr int WB = Sh::WB(r), K = Sh::K(r);
+    if constexpr (WB >= 4) {
+      constexpr int QB = S - 4 - WB;  // bits of q above the window
+      const u32 idx = t >> K, i = t & ((1u << K) - 1);
+      if (idx < (1u << (8 - WB))) {
+        const u32 cc = idx >> QB, qq = (idx & ((1u << QB) - 1)) << WB;
+        const u32 hb = (u32)((((tile0 << LOG_C) + cc) & ((1ull << chunk_bits) - 1)) << S);
+        const u32 e = round_exp(a, hb + window_a<WB>(qq, 0), Sh::AB(r), K);
+        ltw[r][t] = e ? pow_w(a.tab, e * i, INV) : 1;  // W^i: omega has order 2^30 | 2^32
+      }
+    } else {
+      const u64 w = round_twiddle(a, hbase + window_a<WB>(q, 0), Sh::AB(r), K, INV);
+      twiddle_powers<K>(tw[r], w);
+    }
   });
   const bool top = S == L;  // single pass: the top round has no twiddle
   u64 fac[VALS];
@@ -490,7 +510,18 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
           v[m] = mul(v[m], fac[m]);
         });
       }
-      do_round<Sh::K(r), N2B, INV>(v, tw[r], !(top && r == Sh::TOP));
+      if constexpr (WB >= 4) {
+        constexpr int K = Sh::K(r), QB = S - 4 - WB;
+        const u32 base = ((c << QB) | (q >> WB)) << K;
+        u64 pw[VALS];
+        static_for<(1 << K)>([&](auto i_) {
+          constexpr int i = decltype(i_)::value;
+          pw[i] = ltw[r][base + i];
+        });
+        do_round<K, N2B, INV>(v, pw, !(top && r == Sh::TOP));
+      } else {
+        do_round<Sh::K(r), N2B, INV>(v, tw[r], !(top && r == Sh::TOP));
+      }
       if (SCALE && INV && r == R - 1) {
         static_for<VALS>([&](auto m_) {
           constexpr int m = decltype(m_)::value;
@@ -579,12 +610,17 @@ __global__ __launch_bounds__(THREADS) void ntt_pass_strided_group(const PassArgs
   const u32 j = group_of(wg_start, njobs, blockIdx.x);
   strided_body<S, N2B, INV, TOP, SCALE, GATHER>(jobs[j], blockIdx.x - wg_start[j]);
 }
+// Waves per SIMD the contiguous pass is compiled for: 3 fits its registers once the upper
+// rounds' twiddles live in LDS; the fused-scale variants keep 16 more values per thread.
+#ifndef BA_CONTIG_WAVES
+#define BA_CONTIG_WAVES 3
+#endif
 template <int S, bool N2B, bool INV, bool SCALE, bool WHOLE>
-__global__ __launch_bounds__(THREADS) void ntt_pass_contig(PassArgs a) {
+__global__ __launch_bounds__(THREADS, SCALE ? 2 : BA_CONTIG_WAVES) void ntt_pass_contig(PassArgs a) {
   contig_body<S, N2B, INV, SCALE, WHOLE>(a, blockIdx.x);
 }
 template <int S, bool N2B, bool INV, bool SCALE, bool WHOLE>
-__global__ __launch_bounds__(THREADS) void ntt_pass_contig_group(const PassArgs *jobs, const u32 *wg_start,
+__global__ __launch_bounds__(THREADS, SCALE ? 2 : BA_CONTIG_WAVES) void ntt_pass_contig_group(const PassArgs *jobs, const u32 *wg_start,
                                                                  u32 njobs) {
   const u32 j = group_of(wg_start, njobs, blockIdx.x);
   contig_body<S, N2B, INV, SCALE, WHOLE>(jobs[j], blockIdx.x - wg_start[j]);
