@@ -152,6 +152,60 @@ int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_stride, uint64_t
                  uint64_t *out, uint64_t out_col_stride, uint64_t out_coset_stride, uint32_t log_n,
                  uint32_t num_cols, uint32_t log_blowup, ba_stream_t stream);
 
+/* ---------------------------------------------------------------- powers, omega shifts
+ * boojum_cuda::ops_complex (crates/boojum-cuda/src/ops_complex.rs:31-260): vectors of powers
+ * read from the current device's context tables, same semantics and u32 index wrapping as the
+ * reference kernels (crates/boojum-cuda/native/ops_complex.cu:8-86):
+ *   get_powers_of_w: result[i] = omega_{2^30}^(+-(j << (30 - log_degree))), j = i + offset, or its
+ *     low log_degree bits reversed when bit_reverse (omega_{2^log_degree}^j for j < 2^log_degree);
+ *   get_powers_of_g: 7^(the same exponent mod 2^30), 7^-1 when inverse;
+ *   get_powers_by_val/ref_bf: result[i] = base^((bit_reverse ? brev32(i) : i) + offset);
+ *   get_powers_by_val/ref_ef: the same in F_p[u]/(u^2 - 7), stored as two planes:
+ *     result[i] = c0, result[count + i] = c1 (the reference's extension_field layout);
+ *   omega_shift: dst[i] = src[i] * omega_{2^30}^(+-((shift * (i + offset)) << (30 - log_degree)));
+ *     dst == src is the in-place form (ops_complex.rs:246-260); other overlaps are rejected.
+ * log_degree <= 30. */
+int ba_get_powers_of_w(uint32_t log_degree, uint32_t offset, int inverse, int bit_reverse,
+                       uint64_t *result, uint32_t count, ba_stream_t stream);
+int ba_get_powers_of_g(uint32_t log_degree, uint32_t offset, int inverse, int bit_reverse,
+                       uint64_t *result, uint32_t count, ba_stream_t stream);
+int ba_get_powers_by_val_bf(uint64_t base, uint32_t offset, int bit_reverse, uint64_t *result,
+                            uint32_t count, ba_stream_t stream);
+int ba_get_powers_by_ref_bf(const uint64_t *base, uint32_t offset, int bit_reverse,
+                            uint64_t *result, uint32_t count, ba_stream_t stream);
+int ba_get_powers_by_val_ef(uint64_t base_c0, uint64_t base_c1, uint32_t offset, int bit_reverse,
+                            uint64_t *result, uint32_t count, ba_stream_t stream);
+int ba_get_powers_by_ref_ef(const uint64_t *base, uint32_t offset, int bit_reverse,
+                            uint64_t *result, uint32_t count, ba_stream_t stream);
+int ba_omega_shift(const uint64_t *src, uint32_t log_degree, uint32_t offset, int inverse,
+                   uint32_t shift, uint64_t *dst, uint32_t count, ba_stream_t stream);
+int ba_omega_shift_in_place(uint64_t *values, uint32_t log_degree, uint32_t offset, int inverse,
+                            uint32_t shift, uint32_t count, ba_stream_t stream);
+
+/* ---------------------------------------------------------------- barycentric evaluation
+ * boojum_cuda::barycentric (crates/boojum-cuda/src/barycentric.rs:83-300): evaluations y_i of
+ * polynomials on the coset domain {coset * w^i : i < count} (w = omega_count, count a power of
+ * two) are evaluated at a point X as f(X) = sum_i L_i y_i, with
+ * L_i = common / (X w^-i - coset) and common = coset (X^count - coset^count) / (count coset^count).
+ * ext = 0: X, common_factor (1 element each) and the count coefficients are base-field;
+ * ext = 1: X and common_factor are (c0, c1) in F_p[u]/(u^2 - 7) and the coefficients are two
+ * planes c0[count], c1[count].  bit_reverse stores the coefficients in bit-reversed order (for
+ * evaluations stored bit-reversed, as the LDE leaves them).  x and common_factor are device
+ * pointers (DeviceVariable in the reference); the tables are the current device's context.
+ * batch_eval kind: 0 base-field ys at a base-field X, 1 base-field ys at an extension X,
+ * 2 extension ys (column k at ys + 2 k count: planes c0, c1) at an extension X.  ys columns are
+ * count apart (count for kind 2: planes).  evals: num_polys base elements (kind 0) or
+ * num_polys (c0, c1) pairs.  temp: device scratch of ba_barycentric_batch_eval_temp_elems
+ * elements (the reference's temp_storage arguments, barycentric.rs:193-209). */
+int ba_barycentric_precompute_lagrange_coeffs(int ext, const uint64_t *x, uint64_t *common_factor,
+                                              uint64_t coset, uint64_t *lagrange_coeffs,
+                                              uint32_t count, int bit_reverse, ba_stream_t stream);
+int ba_barycentric_batch_eval_temp_elems(int kind, uint32_t count, uint32_t num_polys,
+                                         uint64_t *elems);
+int ba_barycentric_batch_eval(int kind, const uint64_t *ys, uint32_t count, uint32_t num_polys,
+                              const uint64_t *lagrange_coeffs, uint64_t *temp, uint64_t temp_elems,
+                              uint64_t *evals, ba_stream_t stream);
+
 /* ---------------------------------------------------------------- uneven batches
  * No reference counterpart (boojum-cuda runs one transform per call, ntt.rs:179-291): a list of
  * independent jobs -- batch NTTs of any size, direction and coset, and coset LDEs -- planned

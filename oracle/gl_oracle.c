@@ -244,6 +244,148 @@ int gl_oracle_batch_ntt(const uint64_t *in, uint64_t *out, unsigned log_n, unsig
   return 0;
 }
 
+/* ---- powers and omega shifts (crates/boojum-cuda/native/ops_complex.cu:8-86 semantics) ---- */
+
+/* j = bitrev over the low `bits` bits of x (x >> 32-bits of the 32-bit reversal; 0 for bits 0) */
+static uint32_t rev_low(uint32_t x, unsigned bits) { return bits ? (uint32_t)gl_bitrev(x & (uint32_t)((1ull << bits) - 1), bits) : 0; }
+
+static uint32_t brev32(uint32_t x) { return (uint32_t)gl_bitrev(x, 32); }
+
+void gl_oracle_powers_of_w(unsigned log_degree, uint32_t offset, int inverse, int bit_reverse,
+                           uint64_t *result, uint32_t count) {
+  const uint64_t w = gl_omega(GL_OMEGA_LOG_ORDER);
+  for (uint32_t i = 0; i < count; i++) {
+    const uint32_t raw = i + offset;
+    const uint32_t j = bit_reverse ? rev_low(raw, log_degree) : raw;
+    uint32_t e = (j << (GL_OMEGA_LOG_ORDER - log_degree)) & ((1u << GL_OMEGA_LOG_ORDER) - 1);
+    if (inverse) e = (0u - e) & ((1u << GL_OMEGA_LOG_ORDER) - 1);
+    result[i] = gl_pow(w, e);
+  }
+}
+
+void gl_oracle_powers_of_g(unsigned log_degree, uint32_t offset, int inverse, int bit_reverse,
+                           uint64_t *result, uint32_t count) {
+  const uint64_t g = inverse ? gl_inv(gl_generator()) : gl_generator();
+  for (uint32_t i = 0; i < count; i++) {
+    const uint32_t raw = i + offset;
+    const uint32_t j = bit_reverse ? rev_low(raw, log_degree) : raw;
+    const uint32_t e = (j << (GL_OMEGA_LOG_ORDER - log_degree)) & ((1u << GL_OMEGA_LOG_ORDER) - 1);
+    result[i] = gl_pow(g, e);
+  }
+}
+
+void gl_oracle_powers_by_val_bf(uint64_t base, uint32_t offset, int bit_reverse, uint64_t *result,
+                                uint32_t count) {
+  for (uint32_t i = 0; i < count; i++) result[i] = gl_pow(base, (uint32_t)((bit_reverse ? brev32(i) : i) + offset));
+}
+
+/* F_p[u] / (u^2 - 7) */
+static void ext_mul(const uint64_t x[2], const uint64_t y[2], uint64_t r[2]) {
+  const uint64_t a = gl_add(gl_mul(x[0], y[0]), gl_mul(7, gl_mul(x[1], y[1])));
+  const uint64_t b = gl_add(gl_mul(x[0], y[1]), gl_mul(x[1], y[0]));
+  r[0] = a;
+  r[1] = b;
+}
+
+void gl_oracle_powers_by_val_ef(uint64_t base_c0, uint64_t base_c1, uint32_t offset, int bit_reverse,
+                                uint64_t *result, uint32_t count) {
+  for (uint32_t i = 0; i < count; i++) {
+    uint32_t e = (bit_reverse ? brev32(i) : i) + offset;
+    uint64_t r[2] = {1, 0}, x[2] = {gl_reduce(base_c0), gl_reduce(base_c1)};
+    while (e) {
+      if (e & 1) ext_mul(r, x, r);
+      ext_mul(x, x, x);
+      e >>= 1;
+    }
+    result[i] = r[0];
+    result[(size_t)count + i] = r[1];
+  }
+}
+
+void gl_oracle_omega_shift(const uint64_t *src, unsigned log_degree, uint32_t offset, int inverse,
+                           uint32_t shift, uint64_t *dst, uint32_t count) {
+  const uint64_t w = gl_omega(GL_OMEGA_LOG_ORDER);
+  for (uint32_t i = 0; i < count; i++) {
+    uint32_t e = ((shift * (i + offset)) << (GL_OMEGA_LOG_ORDER - log_degree)) & ((1u << GL_OMEGA_LOG_ORDER) - 1);
+    if (inverse) e = (0u - e) & ((1u << GL_OMEGA_LOG_ORDER) - 1);
+    dst[i] = gl_mul(gl_reduce(src[i]), gl_pow(w, e));
+  }
+}
+
+/* ---- barycentric evaluation (crates/boojum-cuda/native/barycentric.cu:11-158 semantics) ---- */
+
+static void ext_inv(const uint64_t x[2], uint64_t r[2]) {
+  const uint64_t norm = gl_sub(gl_mul(x[0], x[0]), gl_mul(7, gl_mul(x[1], x[1])));
+  const uint64_t ni = gl_inv(norm);
+  r[0] = gl_mul(x[0], ni);
+  r[1] = gl_sub(0, gl_mul(x[1], ni));
+}
+
+static void ext_pow_u64(const uint64_t x[2], uint64_t e, uint64_t r[2]) {
+  uint64_t a[2] = {1, 0}, b[2] = {x[0], x[1]};
+  while (e) {
+    if (e & 1) ext_mul(a, b, a);
+    ext_mul(b, b, b);
+    e >>= 1;
+  }
+  r[0] = a[0];
+  r[1] = a[1];
+}
+
+void gl_oracle_barycentric_coeffs(int ext, uint64_t x0, uint64_t x1, uint64_t coset, uint32_t count,
+                                  int bit_reverse, uint64_t *coeffs) {
+  unsigned log_count = 0;
+  while ((1ull << log_count) < count) log_count++;
+  const uint64_t c = gl_reduce(coset);
+  const uint64_t x[2] = {gl_reduce(x0), ext ? gl_reduce(x1) : 0};
+  /* common = c (X^N - c^N) / (N c^N) */
+  const uint64_t cn = gl_pow(c, count);
+  uint64_t xn[2];
+  ext_pow_u64(x, count, xn);
+  const uint64_t scale = gl_mul(c, gl_inv(gl_mul(count, cn)));
+  const uint64_t common[2] = {gl_mul(gl_sub(xn[0], cn), scale), gl_mul(xn[1], scale)};
+  const uint64_t w_inv = gl_inv(gl_omega(log_count));
+  uint64_t wi = 1; /* w^-i */
+  for (uint32_t i = 0; i < count; i++) {
+    const uint64_t den[2] = {gl_sub(gl_mul(x[0], wi), c), gl_mul(x[1], wi)};
+    uint64_t inv[2], l[2];
+    ext_inv(den, inv);
+    ext_mul(common, inv, l);
+    const uint32_t at = bit_reverse ? (uint32_t)gl_bitrev(i, log_count) : i;
+    coeffs[at] = l[0];
+    if (ext) coeffs[(size_t)count + at] = l[1];
+    wi = gl_mul(wi, w_inv);
+  }
+}
+
+void gl_oracle_barycentric_eval(int kind, const uint64_t *ys, uint32_t count, uint32_t num_polys,
+                                const uint64_t *coeffs, uint64_t *evals) {
+  for (uint32_t k = 0; k < num_polys; k++) {
+    uint64_t s[2] = {0, 0};
+    for (uint32_t i = 0; i < count; i++) {
+      const uint64_t l[2] = {coeffs[i], kind ? coeffs[(size_t)count + i] : 0};
+      uint64_t y[2];
+      if (kind == 2) {
+        y[0] = gl_reduce(ys[(size_t)k * 2 * count + i]);
+        y[1] = gl_reduce(ys[(size_t)k * 2 * count + count + i]);
+      } else {
+        y[0] = gl_reduce(ys[(size_t)k * count + i]);
+        y[1] = 0;
+      }
+      uint64_t t[2];
+      ext_mul(l, y, t);
+      s[0] = gl_add(s[0], t[0]);
+      s[1] = gl_add(s[1], t[1]);
+    }
+    if (kind == 0) {
+      evals[k] = s[0];
+    } else {
+      evals[2 * (size_t)k] = s[0];
+      evals[2 * (size_t)k + 1] = s[1];
+    }
+  }
+}
+
 void gl_oracle_bit_reverse(const uint64_t *src, uint64_t src_stride, uint64_t *dst,
                            uint64_t dst_stride, unsigned log_rows, unsigned cols) {
   size_t n = (size_t)1 << log_rows;

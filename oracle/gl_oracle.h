@@ -75,6 +75,42 @@ int gl_oracle_batch_ntt(const uint64_t *in, uint64_t *out, unsigned log_n, unsig
                         uint64_t stride_in, uint64_t stride_out, int bitrev_inputs, int inverse,
                         unsigned log_ext, unsigned coset_idx, int nthreads);
 
+/*
+ * Vectors of powers and omega shifts, boojum_cuda::ops_complex (crates/boojum-cuda/src/
+ * ops_complex.rs:31-260; kernels crates/boojum-cuda/native/ops_complex.cu:8-86), u32 index
+ * arithmetic wrapping as there.  Computed with gl_pow from the definitions, no tables.
+ *   powers_of_w: r[i] = omega_{2^30}^(+-(j << (30 - log_degree)) mod 2^30),
+ *                j = bit_reverse ? bitrev_{log_degree}(low log_degree bits of i + offset) : i + offset
+ *   powers_of_g: r[i] = 7^(that exponent mod 2^30), or (7^-1)^(...) when inverse
+ *   powers_by_val_bf: r[i] = base^((bit_reverse ? brev32(i) : i) + offset mod 2^32)
+ *   powers_by_val_ef: the same in F_p[u]/(u^2 - 7); r[i] = c0, r[count + i] = c1
+ *   omega_shift: dst[i] = src[i] * omega_{2^30}^(+-((shift * (i + offset)) << (30 - log_degree)))
+ */
+void gl_oracle_powers_of_w(unsigned log_degree, uint32_t offset, int inverse, int bit_reverse,
+                           uint64_t *result, uint32_t count);
+void gl_oracle_powers_of_g(unsigned log_degree, uint32_t offset, int inverse, int bit_reverse,
+                           uint64_t *result, uint32_t count);
+void gl_oracle_powers_by_val_bf(uint64_t base, uint32_t offset, int bit_reverse, uint64_t *result,
+                                uint32_t count);
+void gl_oracle_powers_by_val_ef(uint64_t base_c0, uint64_t base_c1, uint32_t offset, int bit_reverse,
+                                uint64_t *result, uint32_t count);
+void gl_oracle_omega_shift(const uint64_t *src, unsigned log_degree, uint32_t offset, int inverse,
+                           uint32_t shift, uint64_t *dst, uint32_t count);
+
+/*
+ * Barycentric evaluation (crates/boojum-cuda/src/barycentric.rs:83-300, kernels
+ * crates/boojum-cuda/native/barycentric.cu:11-158), from the formula with one inversion per
+ * element: L_i = coset (X^N - coset^N) / (N coset^N) / (X w^-i - coset), w = omega_N; ext selects
+ * X = x0 + x1 u in F_p[u]/(u^2 - 7) and two-plane coefficients; bit_reverse permutes them.
+ * eval kind 0: base ys at base X; 1: base ys at extension X; 2: extension ys (column k at
+ * 2 k count, planes c0 and c1) at extension X.  evals: kind 0 one element per column, else
+ * (c0, c1) interleaved.
+ */
+void gl_oracle_barycentric_coeffs(int ext, uint64_t x0, uint64_t x1, uint64_t coset, uint32_t count,
+                                  int bit_reverse, uint64_t *coeffs);
+void gl_oracle_barycentric_eval(int kind, const uint64_t *ys, uint32_t count, uint32_t num_polys,
+                                const uint64_t *coeffs, uint64_t *evals);
+
 /* per-column bit reversal (crates/boojum-cuda/src/ops_complex.rs:262-331) */
 void gl_oracle_bit_reverse(const uint64_t *src, uint64_t src_stride, uint64_t *dst,
                            uint64_t dst_stride, unsigned log_rows, unsigned cols);

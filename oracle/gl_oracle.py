@@ -47,6 +47,13 @@ def lib():
             ("gl_oracle_bit_reverse", None, [_u64p, u64, _u64p, u64, u32, u32]),
             ("gl_oracle_lde", i32, [_u64p, u64, _u64p, u64, u64, u32, u32, u32, i32]),
             ("gl_fill_uniform", None, [_u64p, ctypes.c_size_t, u64]),
+            ("gl_oracle_powers_of_w", None, [u32, u32, i32, i32, _u64p, u32]),
+            ("gl_oracle_powers_of_g", None, [u32, u32, i32, i32, _u64p, u32]),
+            ("gl_oracle_powers_by_val_bf", None, [u64, u32, i32, _u64p, u32]),
+            ("gl_oracle_powers_by_val_ef", None, [u64, u64, u32, i32, _u64p, u32]),
+            ("gl_oracle_omega_shift", None, [_u64p, u32, u32, i32, u32, _u64p, u32]),
+            ("gl_oracle_barycentric_coeffs", None, [i32, u64, u64, u64, u32, i32, _u64p]),
+            ("gl_oracle_barycentric_eval", None, [i32, _u64p, u32, u32, _u64p, _u64p]),
         ]:
             f = getattr(L, name)
             f.restype = res
@@ -119,3 +126,51 @@ def fill_uniform(n, seed):
     v = np.empty(n, dtype=np.uint64)
     lib().gl_fill_uniform(_ptr(v), n, seed)
     return v
+
+
+def powers_of_w(log_degree, offset, inverse, bit_reverse, count):
+    r = np.empty(count, dtype=np.uint64)
+    lib().gl_oracle_powers_of_w(log_degree, offset, int(inverse), int(bit_reverse), _ptr(r), count)
+    return r
+
+
+def powers_of_g(log_degree, offset, inverse, bit_reverse, count):
+    r = np.empty(count, dtype=np.uint64)
+    lib().gl_oracle_powers_of_g(log_degree, offset, int(inverse), int(bit_reverse), _ptr(r), count)
+    return r
+
+
+def powers_by_val_bf(base, offset, bit_reverse, count):
+    r = np.empty(count, dtype=np.uint64)
+    lib().gl_oracle_powers_by_val_bf(base, offset, int(bit_reverse), _ptr(r), count)
+    return r
+
+
+def powers_by_val_ef(base_c0, base_c1, offset, bit_reverse, count):
+    """Two planes: [c0 of every power, c1 of every power]."""
+    r = np.empty(2 * count, dtype=np.uint64)
+    lib().gl_oracle_powers_by_val_ef(base_c0, base_c1, offset, int(bit_reverse), _ptr(r), count)
+    return r
+
+
+def omega_shift(src, log_degree, offset, inverse, shift):
+    src = np.ascontiguousarray(src, dtype=np.uint64)
+    dst = np.empty_like(src)
+    lib().gl_oracle_omega_shift(_ptr(src), log_degree, offset, int(inverse), shift, _ptr(dst), src.size)
+    return dst
+
+
+def barycentric_coeffs(x, coset, count, bit_reverse, ext=False):
+    """x: an int (base field) or a (c0, c1) pair (extension)."""
+    x0, x1 = (x if ext else (x, 0))
+    r = np.empty(count * (2 if ext else 1), dtype=np.uint64)
+    lib().gl_oracle_barycentric_coeffs(int(ext), x0, x1, coset, count, int(bit_reverse), _ptr(r))
+    return r
+
+
+def barycentric_eval(kind, ys, count, num_polys, coeffs):
+    ys = np.ascontiguousarray(ys, dtype=np.uint64)
+    coeffs = np.ascontiguousarray(coeffs, dtype=np.uint64)
+    r = np.empty(num_polys * (1 if kind == 0 else 2), dtype=np.uint64)
+    lib().gl_oracle_barycentric_eval(kind, _ptr(ys), count, num_polys, _ptr(coeffs), _ptr(r))
+    return r

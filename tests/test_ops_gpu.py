@@ -93,3 +93,95 @@ def test_fill_uniform_matches_oracle(ba):
         assert np.all(got < np.uint64(P))
     finally:
         d.free()
+
+
+# ---- powers and omega shifts (ops_complex.rs:31-260) against the oracle ----
+@pytest.fixture(scope="module")
+def tables(ba):
+    """The power tables these calls read: the current device's live context."""
+    ctx = ba.Context.create(12, 12)
+    yield ctx
+    ctx.destroy()
+
+POWER_CASES = [  # (log_degree, offset, inverse, bit_reverse, count)
+    (0, 0, False, False, 5), (0, 3, True, True, 5), (1, 0, False, True, 2), (10, 0, False, False, 1024),
+    (10, 0, True, True, 1024), (16, 12345, False, True, 70000), (20, 2**32 - 1000, True, False, 3000),
+    (30, 7, False, False, 4099), (30, 0, True, True, 1 << 16), (12, 2**31, False, True, 777),
+]
+
+
+def test_powers_of_w_and_g(ba, tables):
+    for ld, off, inv, bitrev, count in POWER_CASES:
+        log(f"powers of w/g log_degree={ld} offset={off} inverse={inv} bitrev={bitrev} count={count}")
+        d = ba.DeviceAllocation(count)
+        try:
+            ba.ops.get_powers_of_w(ld, off, inv, bitrev, d)
+            ba.synchronize()
+            assert np.array_equal(d.to_host(), gl_oracle.powers_of_w(ld, off, inv, bitrev, count))
+            ba.ops.get_powers_of_g(ld, off, inv, bitrev, d)
+            ba.synchronize()
+            assert np.array_equal(d.to_host(), gl_oracle.powers_of_g(ld, off, inv, bitrev, count))
+        finally:
+            d.free()
+
+
+def test_powers_by_val_and_ref(ba):
+    rng = np.random.default_rng(8)
+    for off, bitrev, count in [(0, False, 1), (0, False, 100000), (5, True, 3000), (2**32 - 50, False, 4000),
+                               (123, False, 1 << 20)]:
+        base = int(rng.integers(0, 2**64, dtype=np.uint64))
+        e0, e1 = (int(x) for x in rng.integers(0, 2**64, size=2, dtype=np.uint64))
+        log(f"powers by val/ref offset={off} bitrev={bitrev} count={count}")
+        d = ba.DeviceAllocation(2 * count)
+        b = ba.DeviceAllocation.from_host(np.array([base, 0], dtype=np.uint64))
+        be = ba.DeviceAllocation.from_host(np.array([e0, e1], dtype=np.uint64))
+        try:
+            want = gl_oracle.powers_by_val_bf(base, off, bitrev, count)
+            ba.ops.get_powers_by_val_bf(base, off, bitrev, d.sub(0, count))
+            ba.synchronize()
+            assert np.array_equal(d.to_host(count), want)
+            d.copy_from_host(np.zeros(2 * count, dtype=np.uint64))
+            ba.ops.get_powers_by_ref_bf(b.sub(0, 1), off, bitrev, d.sub(0, count))
+            ba.synchronize()
+            assert np.array_equal(d.to_host(count), want)
+            if count <= 4000:  # the oracle's per-element extension pow is slow in bulk
+                want_e = gl_oracle.powers_by_val_ef(e0, e1, off, bitrev, count)
+                ba.ops.get_powers_by_val_ef((e0, e1), off, bitrev, d)
+                ba.synchronize()
+                assert np.array_equal(d.to_host(), want_e)
+                d.copy_from_host(np.zeros(2 * count, dtype=np.uint64))
+                ba.ops.get_powers_by_ref_ef(be, off, bitrev, d)
+                ba.synchronize()
+                assert np.array_equal(d.to_host(), want_e)
+        finally:
+            for t in (d, b, be):
+                t.free()
+
+
+def test_omega_shift(ba, tables):
+    rng = np.random.default_rng(9)
+    for ld, off, inv, shift, count in [(10, 0, False, 1, 1024), (16, 99, True, 3, 50000),
+                                        (30, 2**32 - 7, False, 2**31 + 1, 5000), (0, 5, False, 7, 9)]:
+        log(f"omega_shift log_degree={ld} offset={off} inverse={inv} shift={shift} count={count}")
+        x = raw_u64(rng, count)
+        want = gl_oracle.omega_shift(x, ld, off, inv, shift)
+        src = ba.DeviceAllocation.from_host(x)
+        dst = ba.DeviceAllocation(count)
+        try:
+            ba.ops.omega_shift(src, ld, off, inv, shift, dst)
+            ba.synchronize()
+            assert np.array_equal(dst.to_host(), want)
+            ba.ops.omega_shift_in_place(src, ld, off, inv, shift)
+            ba.synchronize()
+            assert np.array_equal(src.to_host(), want)
+        finally:
+            src.free()
+            dst.free()
+    a = ba.DeviceAllocation(100)
+    try:
+        with pytest.raises(ba.BoojumAmdError):  # partial overlap
+            ba.check(ba.lib().ba_omega_shift(a.ptr, 5, 0, 0, 1, a.ptr + 8, 50, None))
+        with pytest.raises(ba.BoojumAmdError):
+            ba.ops.get_powers_of_w(31, 0, False, False, a)
+    finally:
+        a.free()

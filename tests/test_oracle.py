@@ -182,3 +182,157 @@ def test_fill_uniform_matches_splitmix():
         while x >= P:
             x = mix(x ^ g)
         assert int(v[i]) == x
+
+
+# ---- powers and omega shifts (crates/boojum-cuda/native/ops_complex.cu:8-86 semantics) ----
+W30 = 0x7E9BD009B86A0845  # omega_{2^30}, closed form pinned in test_roots_closed_form
+
+
+def test_powers_of_w_closed_form_and_group():
+    for ld in (0, 1, 4, 10, 30):
+        count = min(1 << ld, 64) if ld else 3
+        r = gl_oracle.powers_of_w(ld, 0, False, False, count)
+        for i in range(count):
+            assert int(r[i]) == pow(W30, (i << (30 - ld)) % 2**30, P), (ld, i)
+    # the full domain of 2^6: distinct, r[i] r[j] = r[i+j], r[32] = -1, omega_64 = 2^39
+    r = [int(x) for x in gl_oracle.powers_of_w(6, 0, False, False, 64)]
+    assert len(set(r)) == 64 and r[1] == pow(2, 39, P) and r[32] == P - 1
+    assert all(r[i] * r[j] % P == r[(i + j) % 64] for i in range(0, 64, 7) for j in range(0, 64, 5))
+    inv = gl_oracle.powers_of_w(6, 0, True, False, 64)
+    assert all(int(a) * int(b) % P == 1 for a, b in zip(r, inv))
+    br_ = gl_oracle.powers_of_w(6, 0, False, True, 64)
+    assert [int(br_[i]) for i in range(64)] == [r[br.bitrev(i, 6)] for i in range(64)]
+
+
+def test_powers_of_w_offset_and_wrap():
+    # i + offset wraps at 2^32 and, bit-reversed, only its low log_degree bits count
+    off = 2**32 - 5
+    r = gl_oracle.powers_of_w(8, off, False, True, 10)
+    for i in range(10):
+        raw = (i + off) % 2**32
+        j = br.bitrev(raw & 255, 8)
+        assert int(r[i]) == pow(W30, (j << 22) % 2**30, P)
+    r = gl_oracle.powers_of_w(3, 5, True, False, 20)
+    for i in range(20):
+        e = (-(((i + 5) << 27) % 2**32)) % 2**30
+        assert int(r[i]) == pow(W30, e, P)
+
+
+def test_powers_of_g():
+    g_inv = pow(7, P - 2, P)
+    for ld, off, inv, bitrev in [(5, 0, False, False), (5, 3, True, True), (30, 7, False, False), (12, 2**31, True, False)]:
+        r = gl_oracle.powers_of_g(ld, off, inv, bitrev, 40)
+        for i in range(40):
+            raw = (i + off) % 2**32
+            j = br.bitrev(raw & ((1 << ld) - 1), ld) if bitrev else raw
+            e = ((j << (30 - ld)) % 2**32) % 2**30
+            assert int(r[i]) == pow(g_inv if inv else 7, e, P), (ld, off, inv, bitrev, i)
+
+
+def test_powers_by_val_bf():
+    base = 0x1234567890ABCDEF
+    for off, bitrev in [(0, False), (9, False), (2**32 - 3, False), (5, True)]:
+        r = gl_oracle.powers_by_val_bf(base, off, bitrev, 33)
+        for i in range(33):
+            e = ((br.bitrev(i, 32) if bitrev else i) + off) % 2**32
+            assert int(r[i]) == pow(base, e, P), (off, bitrev, i)
+
+
+def ext_mul_py(x, y):
+    return ((x[0] * y[0] + 7 * x[1] * y[1]) % P, (x[0] * y[1] + x[1] * y[0]) % P)
+
+
+def ext_pow_py(x, e):
+    r = (1, 0)
+    while e:
+        if e & 1:
+            r = ext_mul_py(r, x)
+        x = ext_mul_py(x, x)
+        e >>= 1
+    return r
+
+
+def test_powers_by_val_ef():
+    # 7 is a quadratic non-residue, so F_p[u]/(u^2 - 7) is a field (goldilocks_extension.cuh:11)
+    assert pow(7, (P - 1) // 2, P) == P - 1
+    base = (0xDEADBEEF12345678, 0x0123456789ABCDEF)
+    for off, bitrev in [(0, False), (11, True), (2**32 - 1, False)]:
+        r = gl_oracle.powers_by_val_ef(base[0], base[1], off, bitrev, 17)
+        for i in range(17):
+            e = ((br.bitrev(i, 32) if bitrev else i) + off) % 2**32
+            assert (int(r[i]), int(r[17 + i])) == ext_pow_py(base, e), (off, bitrev, i)
+    # u^2 = 7: powers of u alternate between the planes
+    r = gl_oracle.powers_by_val_ef(0, 1, 0, False, 5)
+    assert [(int(r[i]), int(r[5 + i])) for i in range(5)] == [(1, 0), (0, 1), (7, 0), (0, 7), (49, 0)]
+
+
+def test_omega_shift():
+    rng = np.random.default_rng(4)
+    src = rng.integers(0, 2**64, size=50, dtype=np.uint64)
+    # shift 1 is multiplication by the powers of omega (a relation between the two functions)
+    for ld, off, inv in [(10, 0, False), (10, 77, True), (30, 2**32 - 20, False)]:
+        got = gl_oracle.omega_shift(src, ld, off, inv, 1)
+        pw = gl_oracle.powers_of_w(ld, off, inv, False, 50)
+        assert all(int(g) == int(s) % P * int(w) % P for g, s, w in zip(got, src, pw))
+    for shift in (3, 2**31 + 5):
+        got = gl_oracle.omega_shift(src, 12, 9, False, shift)
+        for i in range(50):
+            e = (((shift * (i + 9)) % 2**32) << 18) % 2**32 % 2**30
+            assert int(got[i]) == int(src[i]) % P * pow(W30, e, P) % P
+
+
+# ---- barycentric evaluation (crates/boojum-cuda/native/barycentric.cu:11-158) ----
+def _ext_horner(coeffs, x):
+    acc = (0, 0)
+    for c in reversed(coeffs):
+        acc = ext_mul_py(acc, x)
+        acc = ((acc[0] + c[0]) % P, (acc[1] + c[1]) % P)
+    return acc
+
+
+def test_barycentric_equals_direct_evaluation():
+    """sum_i L_i y_i equals the polynomial's value at X, computed by Horner in big integers,
+    for base and extension points and both coefficient orders."""
+    rng = np.random.default_rng(21)
+
+    def rnd():
+        return int(rng.integers(0, P, dtype=np.uint64))
+
+    for log_n in (0, 1, 4, 6):
+        n = 1 << log_n
+        w = pow(W30, 1 << (30 - log_n), P)
+        coset = 7 * pow(W30, 12345, P) % P
+        mono = [(rnd(), rnd()) for _ in range(n)]  # extension coefficients
+        mono_b = [(m[0], 0) for m in mono]
+        pts = [coset * pow(w, i, P) % P for i in range(n)]
+        ys_e = [_ext_horner(mono, (pt, 0)) for pt in pts]
+        ys_b = [_ext_horner(mono_b, (pt, 0))[0] for pt in pts]
+        xb, xe = rnd(), (rnd(), rnd())
+        for bitrev in (False, True):
+            # storage position q holds the value at domain point bitrev(q) (or q)
+            src = [br.bitrev(q, log_n) if bitrev else q for q in range(n)]
+            yb = np.array([ys_b[i] for i in src], dtype=np.uint64)
+            ye = np.array([ys_e[i][0] for i in src] + [ys_e[i][1] for i in src], dtype=np.uint64)
+            cb = gl_oracle.barycentric_coeffs(xb, coset, n, bitrev)
+            assert int(gl_oracle.barycentric_eval(0, yb, n, 1, cb)[0]) == _ext_horner(mono_b, (xb, 0))[0]
+            ce = gl_oracle.barycentric_coeffs(xe, coset, n, bitrev, ext=True)
+            got = gl_oracle.barycentric_eval(1, yb, n, 1, ce)
+            assert (int(got[0]), int(got[1])) == _ext_horner(mono_b, xe), (log_n, bitrev)
+            got = gl_oracle.barycentric_eval(2, ye, n, 1, ce)
+            assert (int(got[0]), int(got[1])) == _ext_horner(mono, xe), (log_n, bitrev)
+
+
+def test_barycentric_on_lde_cosets():
+    """The interpolant of evaluations on the base domain, evaluated at X from the evaluations
+    themselves and from each bit-reversed LDE coset (shift 7 w_{Bn}^bitrev(c)), agrees."""
+    rng = np.random.default_rng(22)
+    log_n, lb = 10, 2
+    n, B = 1 << log_n, 1 << lb
+    ev = rng.integers(0, P, size=n, dtype=np.uint64)
+    lde = gl_oracle.lde(ev, log_n, 1, lb)  # coset c at [c n, (c + 1) n), bit-reversed
+    x = int(rng.integers(0, P, dtype=np.uint64))
+    want = int(gl_oracle.barycentric_eval(0, ev, n, 1, gl_oracle.barycentric_coeffs(x, 1, n, False))[0])
+    for c in range(B):
+        shift = 7 * pow(W30, br.bitrev(c, lb) << (30 - log_n - lb), P) % P
+        coeffs = gl_oracle.barycentric_coeffs(x, shift, n, True)
+        assert int(gl_oracle.barycentric_eval(0, lde[c * n:(c + 1) * n], n, 1, coeffs)[0]) == want, c

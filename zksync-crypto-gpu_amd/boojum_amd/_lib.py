@@ -90,6 +90,17 @@ SIGNATURES = {
     "ba_batch_plan_stats": [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32), ctypes.POINTER(_u64)],
     "ba_batch_plan_destroy": [_vp],
     "ba_autotune": [_vp, _u32, _u32, _u64, _vp],
+    "ba_get_powers_of_w": [_u32, _u32, _i32, _i32, _vp, _u32, _vp],
+    "ba_get_powers_of_g": [_u32, _u32, _i32, _i32, _vp, _u32, _vp],
+    "ba_get_powers_by_val_bf": [_u64, _u32, _i32, _vp, _u32, _vp],
+    "ba_get_powers_by_ref_bf": [_vp, _u32, _i32, _vp, _u32, _vp],
+    "ba_get_powers_by_val_ef": [_u64, _u64, _u32, _i32, _vp, _u32, _vp],
+    "ba_get_powers_by_ref_ef": [_vp, _u32, _i32, _vp, _u32, _vp],
+    "ba_omega_shift": [_vp, _u32, _u32, _i32, _u32, _vp, _u32, _vp],
+    "ba_omega_shift_in_place": [_vp, _u32, _u32, _i32, _u32, _u32, _vp],
+    "ba_barycentric_precompute_lagrange_coeffs": [_i32, _vp, _vp, _u64, _vp, _u32, _i32, _vp],
+    "ba_barycentric_batch_eval_temp_elems": [_i32, _u32, _u32, ctypes.POINTER(_u64)],
+    "ba_barycentric_batch_eval": [_i32, _vp, _u32, _u32, _vp, _vp, _u64, _vp, _vp],
     "ba_server_create": [_vp, _sz, _u32, ctypes.POINTER(_vp)],
     "ba_server_submit": [_vp, ctypes.POINTER(ServeRequest), ctypes.POINTER(_u64)],
     "ba_server_wait": [_vp, _u64, ctypes.POINTER(ServeResult)],
