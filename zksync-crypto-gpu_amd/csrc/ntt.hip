@@ -81,18 +81,20 @@ namespace {
 
 using namespace gl::ntt;
 
-BaKernels pick_kernel(bool contig, uint32_t S, bool n2b, bool inverse, bool top, bool scale,
+// scale: 0 none, 1 one factor for every element, 2 per-element coset factors (the strided
+// kernels fuse 1 and 2 alike)
+BaKernels pick_kernel(bool contig, uint32_t S, bool n2b, bool inverse, bool top, int scale,
                       bool whole, bool gather) {
   if (contig)
     return S <= 8 ? ba_pick_contig_lo(S, n2b, inverse, scale, whole)
                   : ba_pick_contig_hi(S, n2b, inverse, scale, whole);
-  return ba_pick_strided(S, n2b, inverse, top, scale, gather);
+  return ba_pick_strided(S, n2b, inverse, top, scale != 0, gather);
 }
 
 // kernel names as rocprofv3 prints them, for launch tracing
-const char *kernel_name(bool contig, uint32_t S, bool n2b, bool inverse, bool top, bool scale,
+const char *kernel_name(bool contig, uint32_t S, bool n2b, bool inverse, bool top, int scale,
                         bool gather) {
-  static char names[2][13][2][2][2][2][2][120];
+  static char names[2][13][2][2][2][3][2][120];
   static bool ready = false;
   static const char *tf[2] = {"false", "true"};
   if (!ready) {
@@ -100,14 +102,14 @@ const char *kernel_name(bool contig, uint32_t S, bool n2b, bool inverse, bool to
       for (int d = 0; d < 2; d++)
         for (int i = 0; i < 2; i++)
           for (int tp = 0; tp < 2; tp++)
-            for (int sc = 0; sc < 2; sc++)
+            for (int sc = 0; sc < 3; sc++)
               for (int ga = 0; ga < 2; ga++) {
                 snprintf(names[1][s][d][i][tp][sc][ga], sizeof(names[1][s][d][i][tp][sc][ga]),
-                         "void gl::ntt::ntt_pass_contig<%d, %s, %s, %s, %s>(gl::ntt::PassArgs)", s, tf[d],
-                         tf[i], tf[sc], tf[tp]);
+                         "void gl::ntt::ntt_pass_contig<%d, %s, %s, %d, %s>(gl::ntt::PassArgs)", s, tf[d],
+                         tf[i], sc, tf[tp]);
                 snprintf(names[0][s][d][i][tp][sc][ga], sizeof(names[0][s][d][i][tp][sc][ga]),
                          "void gl::ntt::ntt_pass_strided<%d, %s, %s, %s, %s, %s>(gl::ntt::PassArgs)", s,
-                         tf[d], tf[i], tf[tp], tf[sc], tf[ga]);
+                         tf[d], tf[i], tf[tp], tf[sc != 0], tf[ga]);
               }
     ready = true;
   }
@@ -206,7 +208,7 @@ int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint3
       const uint64_t groups = big ? (nc + a.reps - 1) / a.reps : (tiles + a.reps - 1) / a.reps;
       l.grid = groups << log_stride;
       const bool top = plan.lo_bit[p] + S == log_n;
-      const bool scale = a.scale != SCALE_NONE;
+      const int scale = a.scale == SCALE_NONE ? 0 : (a.scale_coset ? 2 : 1);
       const bool gather = !contig && a.in_bitrev;
       l.kernels = pick_kernel(contig, S, n2b, inverse, top, scale, big, gather);
       l.name = kernel_name(contig, S, n2b, inverse, contig ? big : top, scale, gather);

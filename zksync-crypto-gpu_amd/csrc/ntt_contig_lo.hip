@@ -3,31 +3,35 @@
 
 using namespace gl::ntt;
 
-template <int S, bool N2B, bool INV, bool SCALE, bool WHOLE>
+template <int S, bool N2B, bool INV, int SC, bool WHOLE>
 static BaKernels pair() {
-  return {ntt_pass_contig<S, N2B, INV, SCALE, WHOLE>, ntt_pass_contig_group<S, N2B, INV, SCALE, WHOLE>};
+  return {ntt_pass_contig<S, N2B, INV, SC, WHOLE>, ntt_pass_contig_group<S, N2B, INV, SC, WHOLE>};
 }
 
 // columns of >= 4096 elements use S = 12, or S = 9..11 when log_n is 13..15; shorter
 // columns are transformed in one pass with S = log_n
-template <int S, bool N2B, bool INV, bool SCALE>
+template <int S, bool N2B, bool INV, int SC>
 static BaKernels pick4(bool whole) {
-  if constexpr (S == 12) return pair<S, N2B, INV, SCALE, true>();
-  else if constexpr (S < 9) return pair<S, N2B, INV, SCALE, false>();
-  else return whole ? pair<S, N2B, INV, SCALE, true>() : pair<S, N2B, INV, SCALE, false>();
+  if constexpr (S == 12) return pair<S, N2B, INV, SC, true>();
+  else if constexpr (S < 9) return pair<S, N2B, INV, SC, false>();
+  else return whole ? pair<S, N2B, INV, SC, true>() : pair<S, N2B, INV, SC, false>();
+}
+
+// scale: 0 none, 1 one factor for all elements, 2 per-element coset factors
+template <int S, bool N2B, bool INV>
+static BaKernels pick_sc(int scale, bool whole) {
+  if (scale == 2) return pick4<S, N2B, INV, 2>(whole);
+  if (scale == 1) return pick4<S, N2B, INV, 1>(whole);
+  return pick4<S, N2B, INV, 0>(whole);
 }
 
 template <int S>
-static BaKernels pick(bool n2b, bool inverse, bool scale, bool whole) {
-  if (n2b) {
-    if (inverse) return scale ? pick4<S, true, true, true>(whole) : pick4<S, true, true, false>(whole);
-    return scale ? pick4<S, true, false, true>(whole) : pick4<S, true, false, false>(whole);
-  }
-  if (inverse) return scale ? pick4<S, false, true, true>(whole) : pick4<S, false, true, false>(whole);
-  return scale ? pick4<S, false, false, true>(whole) : pick4<S, false, false, false>(whole);
+static BaKernels pick(bool n2b, bool inverse, int scale, bool whole) {
+  if (n2b) return inverse ? pick_sc<S, true, true>(scale, whole) : pick_sc<S, true, false>(scale, whole);
+  return inverse ? pick_sc<S, false, true>(scale, whole) : pick_sc<S, false, false>(scale, whole);
 }
 
-BaKernels ba_pick_contig_lo(uint32_t S, bool n2b, bool inverse, bool scale, bool whole) {
+BaKernels ba_pick_contig_lo(uint32_t S, bool n2b, bool inverse, int scale, bool whole) {
   switch (S) {
     case 4: return pick<4>(n2b, inverse, scale, whole);
     case 5: return pick<5>(n2b, inverse, scale, whole);

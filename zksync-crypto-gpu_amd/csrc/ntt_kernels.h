@@ -391,11 +391,14 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
   }
 }
 
-// SCALE: fused forward pre-scale (INV = false) or inverse post-scale (INV = true).
+// SC: fused forward pre-scale (INV = false) or inverse post-scale (INV = true): 0 none,
+// 1 one factor for every element (n^-1 of an inverse without coset), 2 per-element coset factors.
 // WHOLE: columns of >= 4096 elements (a tile is one contiguous block of one column); otherwise
 // a tile holds 4096 / 2^S whole columns of 2^S elements.
-template <int S, bool N2B, bool INV, bool SCALE, bool WHOLE>
+template <int S, bool N2B, bool INV, int SC, bool WHOLE>
 __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
+  constexpr bool SCALE = SC == 2;
+  const u64 uniform = a.scale_tab[0];
   using Sh = Shape<S, N2B>;
   constexpr int R = Sh::ROUNDS;
   constexpr int LOG_C = TILE_LOG - S;  // chunks per tile
@@ -504,10 +507,10 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
         v[m] = lds[slot(m)];
       });
       __syncthreads();
-      if (SCALE && !INV && r == 0) {
+      if (SC != 0 && !INV && r == 0) {
         static_for<VALS>([&](auto m_) {
           constexpr int m = decltype(m_)::value;
-          v[m] = mul(v[m], fac[m]);
+          if constexpr (SCALE) v[m] = mul(v[m], fac[m]); else v[m] = mul(v[m], uniform);
         });
       }
       if constexpr (WB >= 4) {
@@ -522,10 +525,10 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
       } else {
         do_round<Sh::K(r), N2B, INV>(v, tw[r], !(top && r == Sh::TOP));
       }
-      if (SCALE && INV && r == R - 1) {
+      if (SC != 0 && INV && r == R - 1) {
         static_for<VALS>([&](auto m_) {
           constexpr int m = decltype(m_)::value;
-          v[m] = mul(v[m], fac[m]);
+          if constexpr (SCALE) v[m] = mul(v[m], fac[m]); else v[m] = mul(v[m], uniform);
         });
       }
       static_for<VALS>([&](auto m_) {
@@ -615,15 +618,15 @@ __global__ __launch_bounds__(THREADS) void ntt_pass_strided_group(const PassArgs
 #ifndef BA_CONTIG_WAVES
 #define BA_CONTIG_WAVES 3
 #endif
-template <int S, bool N2B, bool INV, bool SCALE, bool WHOLE>
-__global__ __launch_bounds__(THREADS, SCALE ? 2 : BA_CONTIG_WAVES) void ntt_pass_contig(PassArgs a) {
-  contig_body<S, N2B, INV, SCALE, WHOLE>(a, blockIdx.x);
+template <int S, bool N2B, bool INV, int SC, bool WHOLE>
+__global__ __launch_bounds__(THREADS, SC == 2 ? 2 : BA_CONTIG_WAVES) void ntt_pass_contig(PassArgs a) {
+  contig_body<S, N2B, INV, SC, WHOLE>(a, blockIdx.x);
 }
-template <int S, bool N2B, bool INV, bool SCALE, bool WHOLE>
-__global__ __launch_bounds__(THREADS, SCALE ? 2 : BA_CONTIG_WAVES) void ntt_pass_contig_group(const PassArgs *jobs, const u32 *wg_start,
+template <int S, bool N2B, bool INV, int SC, bool WHOLE>
+__global__ __launch_bounds__(THREADS, SC == 2 ? 2 : BA_CONTIG_WAVES) void ntt_pass_contig_group(const PassArgs *jobs, const u32 *wg_start,
                                                                  u32 njobs) {
   const u32 j = group_of(wg_start, njobs, blockIdx.x);
-  contig_body<S, N2B, INV, SCALE, WHOLE>(jobs[j], blockIdx.x - wg_start[j]);
+  contig_body<S, N2B, INV, SC, WHOLE>(jobs[j], blockIdx.x - wg_start[j]);
 }
 template <int K, bool N2B, bool INV>
 __global__ __launch_bounds__(THREADS) void ntt_tiny(PassArgs a) {
