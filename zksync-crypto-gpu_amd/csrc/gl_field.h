@@ -43,7 +43,10 @@ __device__ __forceinline__ u64 neg(u64 a) { return a ? P - a : 0; }
 //   c & !b: d + EPS (canonical, since d < hl EPS);  b & !c: d - EPS (canonical);
 //   otherwise d is exact and d >= p selects d + EPS (= d - p mod 2^64).
 // Written in inline asm to use the multiply-add's carry-out and to select with lane masks;
-// the SALU mask logic writes SCC, hence the clobber.
+// the SALU mask logic writes SCC, hence the clobber.  Every carry a VALU instruction writes to
+// an SGPR is read by a VALU instruction at least two instructions later (the VALU-writes-SGPR
+// -> VALU-reads-SGPR wait states of gfx950), so the low halves of d + EPS and d - EPS are
+// formed between the two halves of d.
 __device__ __forceinline__ u64 reduce3(u64 lo, u32 hl, u32 hh) {
   u64 r, c;
   asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=&v"(r), "=&s"(c) : "v"(hl), "v"(lo));
@@ -51,10 +54,10 @@ __device__ __forceinline__ u64 reduce3(u64 lo, u32 hl, u32 hh) {
   u32 d0, d1, t0, t1, u0, u1, o0, o1;
   u64 b, ov, x, m1, m2;
   asm("v_sub_co_u32 %[d0], %[b], %[r0], %[hh]\n\t"
-      "v_subb_co_u32 %[d1], %[b], %[r1], 0, %[b]\n\t"
       "v_add_co_u32 %[t0], %[ov], %[d0], -1\n\t"
-      "v_addc_co_u32 %[t1], %[ov], %[d1], 0, %[ov]\n\t"
       "v_sub_co_u32 %[u0], %[x], %[d0], -1\n\t"
+      "v_subb_co_u32 %[d1], %[b], %[r1], 0, %[b]\n\t"
+      "v_addc_co_u32 %[t1], %[ov], %[d1], 0, %[ov]\n\t"
       "v_subb_co_u32 %[u1], %[x], %[d1], 0, %[x]\n\t"
       "s_andn2_b64 %[m1], %[c], %[b]\n\t"
       "s_xor_b64 %[m2], %[c], %[b]\n\t"
@@ -74,22 +77,20 @@ __device__ __forceinline__ u64 reduce3(u64 lo, u32 hl, u32 hh) {
 }
 
 // Canonical residue of lo + hl (2^32 - 1): r = hl EPS + lo with carry c; when c, r + EPS is
-// canonical (r < hl EPS); otherwise r >= p selects r + EPS (= r - p mod 2^64).
+// canonical (r < hl EPS); otherwise r >= p selects r + EPS (= r - p mod 2^64).  r >= p is the
+// overflow of the 64-bit sum t = r + EPS, found by comparing t < r (no carry chain).
 __device__ __forceinline__ u64 reduce2(u64 lo, u32 hl) {
-  u64 r, c;
+  u64 r, c, m;
   asm("v_mad_u64_u32 %0, %1, %2, -1, %3" : "=&v"(r), "=&s"(c) : "v"(hl), "v"(lo));
-  const u32 r0 = (u32)r, r1 = (u32)(r >> 32);
-  u32 t0, t1, o0, o1;
-  u64 ov, m;
-  asm("v_add_co_u32 %[t0], %[ov], %[r0], -1\n\t"
-      "v_addc_co_u32 %[t1], %[ov], %[r1], 0, %[ov]\n\t"
-      "s_or_b64 %[m], %[c], %[ov]\n\t"
-      "v_cndmask_b32 %[o0], %[r0], %[t0], %[m]\n\t"
-      "v_cndmask_b32 %[o1], %[r1], %[t1], %[m]"
-      : [t0] "=&v"(t0), [t1] "=&v"(t1), [o0] "=&v"(o0), [o1] "=&v"(o1), [ov] "=&s"(ov),
-        [m] "=&s"(m)
-      : [r0] "v"(r0), [r1] "v"(r1), [c] "s"(c)
-      : "scc");
+  const u64 t = r + EPS;
+  asm("v_cmp_lt_u64_e64 %[m], %[t], %[r]\n\t"
+      "s_or_b64 %[m], %[m], %[c]"
+      : [m] "=&s"(m) : [t] "v"(t), [r] "v"(r), [c] "s"(c) : "scc");
+  u32 o0, o1;
+  asm("v_cndmask_b32 %0, %2, %3, %6\n\t"
+      "v_cndmask_b32 %1, %4, %5, %6"
+      : "=&v"(o0), "=&v"(o1)
+      : "v"((u32)r), "v"((u32)t), "v"((u32)(r >> 32)), "v"((u32)(t >> 32)), "s"(m));
   return ((u64)o1 << 32) | o0;
 }
 
