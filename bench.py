@@ -32,8 +32,8 @@ METRIC = "LDE Melem/s + % of HBM roofline, batched vs per-call; serve p50/p99 ms
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=30, help="untimed steps first: the clocks settle after ~30")
     ap.add_argument("--log-n", type=int, default=20)
     ap.add_argument("--cols", type=int, default=64)
     ap.add_argument("--log-blowup", type=int, default=1)
