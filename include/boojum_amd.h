@@ -262,6 +262,9 @@ int ba_autotune(ba_context_t ctx, uint32_t log_n_min, uint32_t log_n_max, uint64
 int ba_tuning_get(ba_context_t ctx, uint32_t log_n, int bitrev_inputs, ba_tuned_plan *plan);
 /* sets (plan != NULL, validated) or clears (plan == NULL) the tuned plan of a size class */
 int ba_tuning_set(ba_context_t ctx, uint32_t log_n, int bitrev_inputs, const ba_tuned_plan *plan);
+/* the splits ba_autotune tries for log_n (host only, no device needed): up to `capacity` plans
+ * written to `plans` (bits set, reps 0); *count receives the total number of candidates */
+int ba_tuning_candidates(uint32_t log_n, ba_tuned_plan *plans, uint32_t capacity, uint32_t *count);
 
 /* ---------------------------------------------------------------- serve mode
  * No reference counterpart (shivini runs one prover's LDEs in program order,

@@ -53,3 +53,50 @@ def test_job_struct_matches_abi():
 
     assert ctypes.sizeof(Job) == 80
     assert [f[0] for f in Job._fields_][:3] == ["kind", "log_n", "num_cols"]
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """sizeof/offsetof of every struct in include/boojum_amd.h, from a C compiler, equal the
+    ctypes mirrors the Python binding passes across the ABI."""
+    import shutil
+    import subprocess
+    from boojum_amd import _lib
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"ba_job": _lib.Job, "ba_tuned_plan": _lib.TunedPlan, "ba_serve_request": _lib.ServeRequest,
+               "ba_serve_result": _lib.ServeResult, "ba_serve_stats": _lib.ServeStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-o", str(exe), str(src)], check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                         text=True).stdout.split("\n") if l)
+    for cname, py in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(py, fname).offset, (cname, fname)
+
+
+def test_tuning_candidates_host_enumeration():
+    """The library's candidate splits (host call, no device) equal the Python mirror's, and every
+    one covers log_n with a 9..12-bit contiguous pass under 4..8-bit strided passes."""
+    from boojum_amd import tune
+
+    for log_n in range(1, 29):
+        lib_c = tune.library_candidates(log_n)
+        assert lib_c == tune.candidate_splits(log_n), log_n
+        for bits in lib_c:
+            assert sum(bits) == log_n
+            if log_n > 12:
+                assert 9 <= bits[0] <= 12 and all(4 <= b <= 8 for b in bits[1:]) and len(bits) <= 8
+            else:
+                assert bits == [log_n]
+    assert [12, 8] in tune.library_candidates(20)  # the built-in 2^20 plan is among them

@@ -110,6 +110,7 @@ SIGNATURES = {
     "ba_server_destroy": [_vp],
     "ba_tuning_get": [_vp, _u32, _i32, ctypes.POINTER(TunedPlan)],
     "ba_tuning_set": [_vp, _u32, _i32, ctypes.POINTER(TunedPlan)],
+    "ba_tuning_candidates": [_u32, ctypes.POINTER(TunedPlan), _u32, ctypes.POINTER(_u32)],
     "ba_profile_begin": [ctypes.POINTER(_vp), _u32],
     "ba_profile_end": [ctypes.POINTER(_u32)],
     "ba_profile_launch_name": [_u32],

@@ -36,6 +36,15 @@ def set_plan(ctx, log_n, bitrev_inputs, bits, reps=None):
     check(lib().ba_tuning_set(ctx.handle, log_n, int(bitrev_inputs), ctypes.byref(p)), "tuning_set")
 
 
+def library_candidates(log_n):
+    """The splits ba_autotune tries, as the library enumerates them (host call, no device)."""
+    n = ctypes.c_uint32(0)
+    check(lib().ba_tuning_candidates(log_n, None, 0, ctypes.byref(n)), "tuning_candidates")
+    arr = (TunedPlan * max(1, n.value))()
+    check(lib().ba_tuning_candidates(log_n, arr, n.value, ctypes.byref(n)), "tuning_candidates")
+    return [list(arr[i].bits[:arr[i].num_passes]) for i in range(n.value)]
+
+
 def candidate_splits(log_n):
     """The splits ba_autotune tries (same enumeration as csrc/tune.hip), low pass first."""
     if log_n <= 12:

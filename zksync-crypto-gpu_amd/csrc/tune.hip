@@ -201,3 +201,15 @@ extern "C" int ba_tuning_set(ba_context_t ctx, uint32_t log_n, int bitrev_inputs
   ctx->tuned[n2b][log_n] = e;
   return BA_SUCCESS;
 }
+
+extern "C" int ba_tuning_candidates(uint32_t log_n, ba_tuned_plan *plans, uint32_t capacity, uint32_t *count) {
+  if (!count || log_n < 1 || log_n > BA_MAX_LOG_N || (capacity && !plans)) return BA_ERROR_INVALID_VALUE;
+  const auto splits = candidate_splits(log_n);
+  *count = (uint32_t)splits.size();
+  for (uint32_t i = 0; i < capacity && i < splits.size(); i++) {
+    plans[i] = ba_tuned_plan{};
+    plans[i].num_passes = (uint32_t)splits[i].size();
+    for (size_t k = 0; k < splits[i].size(); k++) plans[i].bits[k] = splits[i][k];
+  }
+  return BA_SUCCESS;
+}
