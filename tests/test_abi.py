@@ -100,3 +100,17 @@ def test_tuning_candidates_host_enumeration():
             else:
                 assert bits == [log_n]
     assert [12, 8] in tune.library_candidates(20)  # the built-in 2^20 plan is among them
+
+
+def test_rust_sys_crate_declares_header():
+    """The Rust -sys crate (not compiled here: no cargo) declares exactly the header's functions,
+    and its #[repr(C)] structs list the header's fields in order."""
+    src = open(os.path.join(ROOT, "zksync-crypto-gpu_amd", "rust", "boojum-amd-sys", "src", "lib.rs")).read()
+    rust_fns = set(re.findall(r"pub fn (ba_[a-z0-9_]+)\(", src))
+    assert rust_fns == set(declared_functions()), rust_fns ^ set(declared_functions())
+    header = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    for name in ("ba_job", "ba_tuned_plan", "ba_serve_request", "ba_serve_result", "ba_serve_stats"):
+        body = re.search(r"typedef struct \{([^{}]*)\}\s*%s;" % name, header).group(1)
+        c_fields = re.findall(r"(\w+)(?:\[\d+\])?\s*[,;]", body)
+        rust_body = re.search(r"pub struct %s \{(.*?)\}" % name, src, flags=re.S).group(1)
+        assert re.findall(r"pub (\w+):", rust_body) == c_fields, name
