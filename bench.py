@@ -23,6 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "zksync-crypto-gpu_amd"))
 
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_WINST = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions per second on MI355X
 # HBM bytes per launch of each kernel on this workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs scaled by a copy kernel (profiles/lde_64x2p20_blowup2_passes.md)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lde_64x2p20_blowup2_traffic.json")
@@ -361,9 +362,22 @@ def main():
     dom_bytes = 16 * n * cols
     achieved = dom_bytes / (dom_avg_ms / 1e3) / 1e9
     traffic = None
+    valu = None
     if (log_n, cols, lb) == (20, 64, 1) and os.path.exists(TRAFFIC_FILE):
-        t = json.load(open(TRAFFIC_FILE))["traffic_bytes_per_launch"].get(dom_name)
+        prof = json.load(open(TRAFFIC_FILE))
+        t = prof["traffic_bytes_per_launch"].get(dom_name)
         traffic = int(t) if t is not None else None
+        v = prof.get("valu", {}).get(dom_name)
+        if v is not None:
+            # integer VALU work of the same kernel: instructions per element (SQ_INSTS_VALU, one
+            # wave instruction covers 64 lanes) over this run's launch time, against the issue peak
+            # of one wave64 instruction per SIMD every 4 cycles (256 CUs x 4 SIMDs x 2.4 GHz / 4)
+            winst = v["valu_insts_per_elem"] * n * cols / 64
+            achieved_w = winst / (dom_avg_ms / 1e3)
+            valu = {"bound": "valu", "insts_per_elem": v["valu_insts_per_elem"],
+                    "achieved": round(achieved_w / 1e9, 1), "peak": VALU_PEAK_WINST / 1e9,
+                    "unit": "G wave-instr/s", "frac": round(achieved_w / VALU_PEAK_WINST, 4),
+                    "busy_counter": v["valu_busy"]}
 
     per_call = None
     if not args.no_per_call:
@@ -409,6 +423,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "avg_launch_ms": round(dom_avg_ms, 4),
                      "alg_bytes_per_launch": dom_bytes},
+        "valu_roofline": valu,
         "per_kernel_ms": {k: round(v[0] / v[1], 4) for k, v in per_kernel.items()},
         "per_call": per_call,
         "tuning": tuning,
