@@ -425,6 +425,9 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
   // lower window (every thread its own W) keeps its powers in registers, across repeats.
   __shared__ u64 ltw[R][256];
   u64 tw[R][VALS];
+  // an n2b inverse with one factor for every element (n^-1) folds it into the last round's
+  // register twiddles (all 16, W^0 included) instead of a multiply after the round
+  constexpr bool FOLD = N2B && INV && SC == 1 && R >= 2 && Sh::WB(R - 1) < 4;
   static_for<R>([&](auto r_) {
     constexpr int r = decltype(r_)::value;
     constexpr int WB = Sh::WB(r), K = Sh::K(r);
@@ -440,6 +443,12 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
     } else {
       const u64 w = round_twiddle(a, hbase + window_a<WB>(q, 0), Sh::AB(r), K, INV);
       twiddle_powers<K>(tw[r], w);
+      if constexpr (FOLD && r == R - 1) {
+        static_for<(1 << K)>([&](auto i_) {
+          constexpr int i = decltype(i_)::value;
+          tw[r][i] = i ? mul(tw[r][i], uniform) : uniform;
+        });
+      }
     }
   });
   const bool top = S == L;  // single pass: the top round has no twiddle
@@ -522,10 +531,17 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
           pw[i] = ltw[r][base + i];
         });
         do_round<K, N2B, INV>(v, pw, !(top && r == Sh::TOP));
+      } else if constexpr (FOLD && r == R - 1) {
+        constexpr int K = Sh::K(r);
+        static_for<VALS>([&](auto m_) {
+          constexpr int m = decltype(m_)::value;
+          v[m] = mul(v[m], tw[r][m >> (4 - K)]);
+        });
+        local_round<K, N2B, INV>(v);
       } else {
         do_round<Sh::K(r), N2B, INV>(v, tw[r], !(top && r == Sh::TOP));
       }
-      if (SC != 0 && INV && r == R - 1) {
+      if (SC != 0 && INV && r == R - 1 && !FOLD) {
         static_for<VALS>([&](auto m_) {
           constexpr int m = decltype(m_)::value;
           if constexpr (SCALE) v[m] = mul(v[m], fac[m]); else v[m] = mul(v[m], uniform);
