@@ -171,3 +171,41 @@ def test_lde_rejects_bad_arguments(ba):
     finally:
         d.free()
         o.free()
+
+
+def test_lde_graph_replay(ba):
+    """The LDE captured as a hipGraph (as the benchmark runs it) gives the eager result on every
+    replay, with eager work and synchronisation between replays."""
+    import ctypes
+
+    log_n, cols, lb = 16, 6, 2
+    n, B = 1 << log_n, 4
+    rng = np.random.default_rng(31)
+    ev = raw_u64(rng, n * cols)
+    d_ev, d_mono = ba.DeviceAllocation.from_host(ev), ba.DeviceAllocation(n * cols)
+    d_out, d_eager = ba.DeviceAllocation(B * n * cols), ba.DeviceAllocation(B * n * cols)
+    stream = ba.CudaStream()
+    lib = ba.lib()
+    try:
+        ba.lde.lde_batch(ba._test_ctx, d_ev, log_n, cols, lb, d_eager, monomials=d_mono)
+        ba.synchronize()
+        eager = d_eager.to_host()
+        assert np.array_equal(eager, gl_oracle.lde(ev, log_n, cols, lb, nthreads=8))
+        ba.check(lib.ba_stream_begin_capture(stream.handle), "capture")
+        ba.lde.lde_batch(ba._test_ctx, d_ev, log_n, cols, lb, d_out, monomials=d_mono, stream=stream)
+        graph = ctypes.c_void_p(None)
+        ba.check(lib.ba_stream_end_capture(stream.handle, ctypes.byref(graph)), "end capture")
+        try:
+            for rep in range(4):
+                log(f"lde graph replay {rep}")
+                d_out.copy_from_host(np.zeros(B * n * cols, dtype=np.uint64))  # eager work between replays
+                ba.check(lib.ba_graph_launch(graph, stream.handle), "graph launch")
+                stream.synchronize()
+                assert np.array_equal(d_out.to_host(), eager), rep
+                assert np.array_equal(d_ev.to_host(), ev)  # evaluations kept (monomial buffer given)
+        finally:
+            ba.check(lib.ba_graph_destroy(graph), "graph destroy")
+    finally:
+        for t in (d_ev, d_mono, d_out, d_eager):
+            t.free()
+        stream.destroy()

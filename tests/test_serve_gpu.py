@@ -142,3 +142,35 @@ def test_bad_requests(ba):
         srv.destroy()
     with pytest.raises(ba.BoojumAmdError):
         ba.serve.Server(ba._test_ctx, 1 << 24, max_inflight_batches=0)
+
+
+def test_config_shapes_alone_equals_mixed(ba):
+    """Requests of the serve configuration's shapes (8..16 columns, log_n up to 21, blowup 2, 4
+    and 8, synthetic inputs): each result is bit-identical alone and when all are submitted at
+    once, and a sampled column of each matches the oracle."""
+    shapes = [(12, 16, 3), (21, 8, 1), (18, 12, 2), (14, 9, 3), (20, 10, 2), (16, 16, 1), (19, 8, 3)]
+    srv = ba.serve.Server(ba._test_ctx, 24 << 30, max_inflight_batches=2)
+    try:
+        alone = []
+        for i, (log_n, cols, lb) in enumerate(shapes):
+            res = srv.wait(srv.submit(log_n, cols, lb, seed=7000 + i))
+            assert res["status"] == 0
+            alone.append(srv.output(res, log_n, cols, lb).to_host())
+            srv.release(res["ticket"])
+        tickets = [srv.submit(log_n, cols, lb, seed=7000 + i) for i, (log_n, cols, lb) in enumerate(shapes)]
+        for i, ((log_n, cols, lb), t) in enumerate(zip(shapes, tickets)):
+            res = srv.wait(t)
+            assert res["status"] == 0
+            got = srv.output(res, log_n, cols, lb).to_host()
+            srv.release(t)
+            log(f"serve shape log_n={log_n} cols={cols} blowup={1 << lb}: batch {res['batch']}")
+            assert np.array_equal(got, alone[i]), (log_n, cols, lb)
+            n, k = 1 << log_n, cols - 1
+            x = gl_oracle.fill_uniform(n, column_seed(7000 + i, k))
+            want = gl_oracle.lde(x, log_n, 1, lb, nthreads=8)
+            assert np.array_equal(got[k * (n << lb):(k + 1) * (n << lb)], want), (log_n, cols, lb)
+        st = srv.stats()
+        log(f"stats {st}")
+        assert st["failed"] == 0 and st["in_use_bytes"] == 0
+    finally:
+        srv.destroy()
