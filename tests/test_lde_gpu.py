@@ -168,9 +168,36 @@ def test_lde_rejects_bad_arguments(ba):
                 ba.lde.lde_batch(ba._test_ctx, d, log_n, 1, lb, o)
         with pytest.raises(ba.BoojumAmdError):
             ba.lde.lde_batch(ba._test_ctx, d, 5, 1, 1, d)  # output aliases evaluations
+        with pytest.raises(ba.BoojumAmdError):  # coset 1 of column 0 is column 1's coset 0
+            ba.lde.lde_batch(ba._test_ctx, d, 5, 2, 1, o, out_col_stride=32, out_coset_stride=32)
+        with pytest.raises(ba.BoojumAmdError):  # output overlaps the evaluations' tail
+            ba.lde.lde_batch(ba._test_ctx, d, 5, 2, 1, d.sub(40, 900))
+        with pytest.raises(ba.BoojumAmdError):  # monomials overlap the evaluations
+            ba.lde.lde_batch(ba._test_ctx, d, 5, 2, 1, o, monomials=d.sub(16, 64))
     finally:
         d.free()
         o.free()
+
+
+def test_context_outlives_its_users(ba):
+    """A context is not destroyed while a batch plan or a server still reads its tables."""
+    ctx = ba.Context.create(12, 12)
+    d = ba.DeviceAllocation(1 << 12)
+    try:
+        plan = ba.batch.BatchPlan(ctx, [ba.batch.ntt_job(d, d, 12, 1, 1 << 12, 1 << 12)])
+        with pytest.raises(ba.BoojumAmdError):
+            ctx.destroy()
+        assert ctx.handle.value  # a refused destroy leaves the context usable
+        plan.launch()
+        ba.synchronize()
+        plan.destroy()
+        srv = ba.serve.Server(ctx, 1 << 24)
+        with pytest.raises(ba.BoojumAmdError):
+            ctx.destroy()
+        srv.destroy()
+    finally:
+        d.free()
+        ctx.destroy()
 
 
 def test_lde_graph_replay(ba):

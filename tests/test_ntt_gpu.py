@@ -215,6 +215,8 @@ def test_invalid_arguments_return_error(ba):
             assert e.value.code == 1
         with pytest.raises(ba.BoojumAmdError):
             ba.ntt.batch_ntt_out_of_place(a, a, 4, 1, 0, 0, 16, 16, False, False, 0, 0)  # aliasing
+        with pytest.raises(ba.BoojumAmdError):  # overlapping (not identical) slices
+            ba.ntt.batch_ntt_out_of_place(a.sub(0, 32), a.sub(16, 32), 4, 1, 0, 0, 16, 16, False, False, 0, 0)
         with pytest.raises(ba.BoojumAmdError):
             ba.ntt.batch_ntt_in_place(a, 4, 5, 0, 16, False, False, 0, 0)  # slice too short
         with pytest.raises(ba.BoojumAmdError):

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "../../include/boojum_amd.h"
 #include "gl_tables.h"
 
@@ -30,6 +32,8 @@ struct ba_context {
   size_t bytes;
   gl::Tables tab;
   BaTuned tuned[2][BA_MAX_LOG_N + 1] = {};  // [n2b][log_n]
+  // batch plans and servers hold pointers into the tables: the context outlives them
+  std::atomic<int> users{0};
 };
 
 // The most recently created live context of `device`, or null.

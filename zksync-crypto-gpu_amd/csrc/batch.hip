@@ -24,6 +24,7 @@
 static_assert(sizeof(ba_job) == 80, "ba_job layout is part of the ABI");
 
 struct ba_batch_plan {
+  ba_context *ctx = nullptr;  // its tables are read by the plan's launches
   int device = 0;
   void *mem = nullptr;  // the uploaded image: PassArgs table, column pointer lists, prefix table
   BaPlanImage image;    // groups ordered by level, largest group first within a level
@@ -34,6 +35,7 @@ struct ba_batch_plan {
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> done;  // one per group
   ~ba_batch_plan() {
+    if (ctx) ctx->users.fetch_sub(1);
     for (hipStream_t s : aux)
       if (s) (void)hipStreamDestroy(s);
     if (fork) (void)hipEventDestroy(fork);
@@ -228,6 +230,8 @@ extern "C" int ba_batch_plan_create(ba_context_t ctx, const ba_job *jobs, uint32
   *plan = nullptr;
   if (!ctx) return BA_ERROR_NOT_INITIALIZED;
   ba_batch_plan *p = new ba_batch_plan();
+  p->ctx = ctx;
+  ctx->users.fetch_add(1);
   (void)hipGetDevice(&p->device);
   int rc = ba_plan_image_build(ctx, jobs, num_jobs, p->image);
   if (rc) {

@@ -114,6 +114,10 @@ extern "C" int ba_context_destroy(ba_context_t ctx) {
   if (!ctx) return BA_ERROR_INVALID_VALUE;
   {
     std::lock_guard<std::mutex> lock(g_registry_mutex);
+    // refused while a batch plan or server still uses its tables (checked before the lookup so
+    // a refused destroy leaves the context fully usable)
+    for (ba_context *c : g_live)
+      if (c == ctx && ctx->users.load() > 0) return BA_ERROR_INVALID_VALUE;
     bool found = false;
     for (auto it = g_live.begin(); it != g_live.end(); ++it)
       if (*it == ctx) {

@@ -9,6 +9,9 @@
 // first pass of every coset NTT gathers them (coalesced chunk reads staged through LDS), so the
 // separate bit-reverse pass over the data disappears.  Launches: passes(iNTT) + B * passes(NTT),
 // all on one stream, capturable into a hipGraph.
+#include <algorithm>
+#include <vector>
+
 #include "api_internal.h"
 
 namespace {
@@ -30,11 +33,29 @@ int ba_validate_lde(uint64_t *evals, uint64_t in_stride, uint64_t *monomials, ui
   const uint32_t blowup = 1u << log_blowup;
   if (num_cols == 0) return BA_SUCCESS;
   if (!evals || !out || in_stride < n) return BA_ERROR_INVALID_VALUE;
-  // cosets of one column must not overlap each other or the next column
-  if (out_coset_stride < n && blowup > 1) return BA_ERROR_INVALID_VALUE;
-  if (num_cols > 1 && out_col_stride < n) return BA_ERROR_INVALID_VALUE;
-  if ((void *)out == (void *)evals || (monomials && (void *)out == (void *)monomials))
-    return BA_ERROR_INVALID_VALUE;
+  // the B * num_cols output vectors of n elements must be pairwise disjoint
+  {
+    std::vector<uint64_t> starts;
+    starts.reserve((size_t)num_cols * blowup);
+    for (uint64_t k = 0; k < num_cols; k++)
+      for (uint64_t c = 0; c < blowup; c++) starts.push_back(k * out_col_stride + c * out_coset_stride);
+    std::sort(starts.begin(), starts.end());
+    for (size_t i = 1; i < starts.size(); i++)
+      if (starts[i] - starts[i - 1] < n) return BA_ERROR_INVALID_VALUE;
+  }
+  // the outputs overlap neither the evaluations nor the monomials; monomials given separately
+  // do not overlap the evaluations
+  const uint64_t *out_end = out + (num_cols - 1) * out_col_stride + (blowup - 1) * out_coset_stride + n;
+  const uint64_t *ev_end = evals + (num_cols - 1) * in_stride + n;
+  auto overlap = [](const uint64_t *a0, const uint64_t *a1, const uint64_t *b0, const uint64_t *b1) {
+    return a0 < b1 && b0 < a1;
+  };
+  if (overlap(out, out_end, evals, ev_end)) return BA_ERROR_INVALID_VALUE;
+  if (monomials) {
+    const uint64_t *mono_end = monomials + (uint64_t)num_cols * n;
+    if (overlap(out, out_end, monomials, mono_end) || overlap(evals, ev_end, monomials, mono_end))
+      return BA_ERROR_INVALID_VALUE;
+  }
   return BA_SUCCESS;
 }
 

@@ -294,7 +294,10 @@ extern "C" int ba_batch_ntt_out_of_place(const uint64_t *inputs_matrix, size_t i
   if ((uint64_t)outputs_offset + n > stride_between_output_arrays) return BA_ERROR_INVALID_VALUE;
   if ((uint64_t)inputs_len < (uint64_t)num_ntts * stride_between_input_arrays) return BA_ERROR_INVALID_VALUE;
   if ((uint64_t)outputs_len < (uint64_t)num_ntts * stride_between_output_arrays) return BA_ERROR_INVALID_VALUE;
-  if ((const void *)inputs_matrix == (const void *)outputs_matrix) return BA_ERROR_INVALID_VALUE;
+  // the reference's slices are distinct borrows (ntt.rs:317,320): no overlap at all
+  if (inputs_len && outputs_len && inputs_matrix < outputs_matrix + outputs_len &&
+      outputs_matrix < inputs_matrix + inputs_len)
+    return BA_ERROR_INVALID_VALUE;
   const uint64_t *in = inputs_matrix + inputs_offset;
   uint64_t *out = outputs_matrix + outputs_offset;
   if ((const void *)in == (const void *)out) return BA_ERROR_INVALID_VALUE;

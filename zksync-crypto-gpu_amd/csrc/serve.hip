@@ -316,6 +316,7 @@ extern "C" int ba_server_create(ba_context_t ctx, size_t memory_budget_bytes, ui
   if (hipGetDevice(&device) != hipSuccess || device != ctx->device) return BA_ERROR_INVALID_VALUE;
   ba_server *s = new ba_server(Arena::round(memory_budget_bytes));
   s->ctx = ctx;
+  ctx->users.fetch_add(1);
   s->device = device;
   hipError_t err = hipMalloc(&s->base, s->arena.size());
   if (err != hipSuccess) s->base = nullptr;
@@ -458,6 +459,7 @@ extern "C" int ba_server_destroy(ba_server_t s) {
     if (sl.desc) (void)hipFree(sl.desc);
   }
   if (s->base) (void)hipFree(s->base);
+  if (s->ctx) s->ctx->users.fetch_sub(1);
   delete s;
   return BA_SUCCESS;
 }
