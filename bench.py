@@ -208,15 +208,30 @@ def run_large(ba, ctx, stream, log_n=22, cols=256, lb=2, steps=3):
         e0.destroy()
         e1.destroy()
         ba.check(lib.ba_graph_destroy(graph), "graph_destroy")
+        # per pass kernel: one traced eager LDE (each launch covers all 256 columns, 2^30 elements)
+        from boojum_amd.profile import LaunchTrace
+        per = {}
+        with LaunchTrace(64) as tr:
+            ba.lde.lde_batch(ctx, d_ev, log_n, cols, lb, d_out, monomials=d_mono, stream=stream)
+        stream.synchronize()
+        for name, kms in tr.durations_ms():
+            tot, cnt = per.get(name, (0.0, 0))
+            per[name] = (tot + kms, cnt + 1)
+        tr.destroy()
     finally:
         for t in (d_ev, d_mono, d_out):
             t.free()
     out_elems = cols * B * n
     alg = 8 * n * (1 + B) * cols
+    pass_bytes = 16 * n * cols  # every pass launch reads and writes each element once
+    kernels = {k: {"avg_launch_ms": round(v[0] / v[1], 3),
+                   "hbm_frac": round(pass_bytes / (v[0] / v[1] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+               for k, v in per.items()}
     return {"workload": f"coset LDE, {cols} columns x 2^{log_n}, blowup {B}, one hipGraph",
             "output_bytes": out_elems * 8, "steps": steps, "ms_per_step": round(ms, 3),
             "melem_s": round(out_elems / (ms / 1e3) / 1e6, 1),
-            "lde_alg_GBs": round(alg / (ms / 1e3) / 1e9, 1), "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+            "lde_alg_GBs": round(alg / (ms / 1e3) / 1e9, 1), "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "pass_kernels": kernels}
 
 
 def run_serve(ba, ctx, n_requests=240, rate_hz=40.0, seed=5, budget_frac=0.6, slots=2):
