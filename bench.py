@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-large", action="store_true", help="skip the 256 x 2^22 blowup-4 leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serve-mode leg")
     ap.add_argument("--no-single", action="store_true", help="skip the single 2^16 NTT leg")
+    ap.add_argument("--no-harness", action="store_true",
+                    help="skip the leg with the reference benchmark's grid (2^28-element matrices)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a 2^27-element copy kernel (canonicalise, 8 B read + 8 B written "
                          "per element) that scales the FETCH_SIZE/WRITE_SIZE counters")
@@ -182,6 +184,57 @@ def run_uneven(ba, ctx, stream, seed=3, jobs_n=512, reps=5):
             "batched_speedup": round(p_ms / b_ms, 3), "plan_launches": st["launches"], "plan_levels": st["levels"]}
 
 
+def run_harness(ba, stream, reps=10, samples=3):
+    """The reference's own benchmark grid (crates/boojum-cuda/benches/ntt.rs:21-126): one
+    2^28-element matrix, transformed out of place as 2^28 / 2^log_n columns by one
+    batch_ntt_out_of_place call, for every (inverse, bitrev_inputs) mode; "batch_ntt" over log_n
+    16..24, "batch_lde" over 16..22 with log_extension_degree 2 and coset_index 1.  Like its
+    criterion harness, each sample times `reps` calls with events on the stream and reports time
+    per individual NTT; here the median of `samples` samples, and the pass traffic (16 B per
+    element) as a fraction of the HBM peak.  Inputs are uniform field elements (the reference
+    feeds non-reduced u64, which the kernels read mod p at the same cost)."""
+    problem = 1 << 28
+    d_in, d_out = ba.DeviceAllocation(problem), ba.DeviceAllocation(problem)
+    e0, e1 = ba.CudaEvent(), ba.CudaEvent()
+    out = {"workload": "2^28-element matrix, out of place, 2^28 / 2^log_n columns per call",
+           "unit": "us per NTT, fraction of HBM peak at 16 B per element"}
+    try:
+        ba.runtime.fill_uniform(d_in, problem, 1, 28)
+        for group, log_ns, lext, coset in (("batch_ntt", range(16, 25), 0, 0), ("batch_lde", range(16, 23), 2, 1)):
+            res = {}
+            for inverse in (False, True):
+                for bitrev in (False, True):
+                    row = {}
+                    for log_n in log_ns:
+                        n = 1 << log_n
+                        cols = problem >> log_n
+
+                        def call():
+                            ba.ntt.batch_ntt_out_of_place(d_in, d_out, log_n, cols, 0, 0, n, n, bitrev, inverse,
+                                                          lext, coset, stream=stream)
+
+                        call()
+                        times = []
+                        for _ in range(samples):
+                            e0.record(stream)
+                            for _ in range(reps):
+                                call()
+                            e1.record(stream)
+                            e1.synchronize()
+                            times.append(e0.elapsed_ms(e1) / reps)
+                        ms = sorted(times)[len(times) // 2]
+                        row[log_n] = [round(ms * 1e3 / cols, 3),
+                                      round(16 * problem / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)]
+                    res[("inverse" if inverse else "forward") + (", b2n" if bitrev else ", n2b")] = row
+            out[group] = res
+    finally:
+        e0.destroy()
+        e1.destroy()
+        d_in.free()
+        d_out.free()
+    return out
+
+
 def run_large(ba, ctx, stream, log_n=22, cols=256, lb=2, steps=3):
     """Large trace LDE: 256 columns of 2^22 at blowup 4 (2^32 output elements, 34.4 GB), the
     whole pipeline captured as one hipGraph, evaluations kept (monomials in their own buffer)."""
@@ -305,6 +358,7 @@ def main():
     ba.set_device(0)
     ctx = ba.Context.create(12, 12)
     log_n, cols, lb = args.log_n, args.cols, args.log_blowup
+    n, B = 1 << log_n, 1 << lb
     tuning = None
     if args.tune_headline:
         t0 = time.perf_counter()
@@ -312,7 +366,6 @@ def main():
         tuning = {"log_n_range": [log_n, log_n], "seconds": round(time.perf_counter() - t0, 2),
                   f"plan_{log_n}_natural": ba.tune.get_plan(ctx, log_n, False),
                   f"plan_{log_n}_bitrev": ba.tune.get_plan(ctx, log_n, True)}
-    n, B = 1 << log_n, 1 << lb
     d_ev = ba.DeviceAllocation(n * cols)
     d_mono = ba.DeviceAllocation(n * cols)
     d_out = ba.DeviceAllocation(B * n * cols)
@@ -460,6 +513,8 @@ def main():
         line["single"] = run_single(ba, ctx, stream)
     if not args.no_large:
         line["large"] = run_large(ba, ctx, stream)
+    if not args.no_harness:
+        line["harness"] = run_harness(ba, stream)
     if not args.no_serve:
         line["serve"] = run_serve(ba, ctx)
     if not args.no_cpu:
