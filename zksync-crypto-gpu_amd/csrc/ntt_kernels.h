@@ -452,15 +452,29 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
     }
   });
   const bool top = S == L;  // single pass: the top round has no twiddle
+  // Per-element coset factors F(j0) T[pi(m)].  n2b pre (single pass) and b2n post (single pass)
+  // act on the window [S-4, S), identity order; n2b post and b2n pre act on the window [0, 4),
+  // natural j = bitrev_L(index).  Over whole 4096-element tiles only F stays in registers and
+  // each factor is formed where it is applied (one more product per element), so the pass
+  // keeps the unscaled pass's three waves per SIMD; tiles of short columns keep all 16 factors
+  // (two waves per SIMD), which spills less there.
+  constexpr bool PERM = N2B != !INV;
+  constexpr bool AT_USE = WHOLE;
+  u64 fbase = 1;
   u64 fac[VALS];
   if constexpr (SCALE) {
-    // n2b pre (single pass) and b2n post (single pass) act on the window [S-4, S), identity order;
-    // n2b post and b2n pre act on the window [0, 4), natural j = bitrev_L(index)
-    if constexpr (N2B == !INV)
-      scale_factors<false>(fac, a, hbase + window_a<S - 4>(q, 0), INV);
+    const u32 j0 = PERM ? bitrev32(hbase + window_a<0>(q, 0), L) : hbase + window_a<S - 4>(q, 0);
+    if constexpr (AT_USE)
+      fbase = scale_base(a, j0, INV);
     else
-      scale_factors<true>(fac, a, bitrev32(hbase + window_a<0>(q, 0), L), INV);
+      scale_factors<PERM>(fac, a, j0, INV);
   }
+  auto factor = [&](int m) -> u64 {
+    if constexpr (AT_USE)
+      return mul(fbase, a.scale_tab[PERM ? cbitrev(m, 4) : m]);
+    else
+      return fac[m];
+  };
 
   // the next tile's elements are loaded while the current one is transformed
   u64 nxt[VALS];
@@ -490,6 +504,8 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
   for (u32 i = 0; i < reps; i++) {
     const u64 tile = tile0 + ((u64)i << a.log_tile_stride);
     const u64 g0 = tile << LOG_C;
+    // keeps the 16 factors from being hoisted out of the loop (into 32 live registers)
+    if constexpr (SCALE && AT_USE) asm volatile("" : "+v"(fbase));
     static_for<VALS>([&](auto k_) {
       constexpr int k = decltype(k_)::value;
       const u32 e = t + THREADS * k;
@@ -519,7 +535,7 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
       if (SC != 0 && !INV && r == 0) {
         static_for<VALS>([&](auto m_) {
           constexpr int m = decltype(m_)::value;
-          if constexpr (SCALE) v[m] = mul(v[m], fac[m]); else v[m] = mul(v[m], uniform);
+          if constexpr (SCALE) v[m] = mul(v[m], factor(m)); else v[m] = mul(v[m], uniform);
         });
       }
       if constexpr (WB >= 4) {
@@ -544,7 +560,7 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
       if (SC != 0 && INV && r == R - 1 && !FOLD) {
         static_for<VALS>([&](auto m_) {
           constexpr int m = decltype(m_)::value;
-          if constexpr (SCALE) v[m] = mul(v[m], fac[m]); else v[m] = mul(v[m], uniform);
+          if constexpr (SCALE) v[m] = mul(v[m], factor(m)); else v[m] = mul(v[m], uniform);
         });
       }
       static_for<VALS>([&](auto m_) {
@@ -630,16 +646,17 @@ __global__ __launch_bounds__(THREADS) void ntt_pass_strided_group(const PassArgs
   strided_body<S, N2B, INV, TOP, SCALE, GATHER>(jobs[j], blockIdx.x - wg_start[j]);
 }
 // Waves per SIMD the contiguous pass is compiled for: 3 fits its registers once the upper
-// rounds' twiddles live in LDS; the fused-scale variants keep 16 more values per thread.
+// rounds' twiddles live in LDS; the per-element-scaled variants over short columns keep 16 more
+// values per thread and run at 2.
 #ifndef BA_CONTIG_WAVES
 #define BA_CONTIG_WAVES 3
 #endif
 template <int S, bool N2B, bool INV, int SC, bool WHOLE>
-__global__ __launch_bounds__(THREADS, SC == 2 ? 2 : BA_CONTIG_WAVES) void ntt_pass_contig(PassArgs a) {
+__global__ __launch_bounds__(THREADS, SC == 2 && !WHOLE ? 2 : BA_CONTIG_WAVES) void ntt_pass_contig(PassArgs a) {
   contig_body<S, N2B, INV, SC, WHOLE>(a, blockIdx.x);
 }
 template <int S, bool N2B, bool INV, int SC, bool WHOLE>
-__global__ __launch_bounds__(THREADS, SC == 2 ? 2 : BA_CONTIG_WAVES) void ntt_pass_contig_group(const PassArgs *jobs, const u32 *wg_start,
+__global__ __launch_bounds__(THREADS, SC == 2 && !WHOLE ? 2 : BA_CONTIG_WAVES) void ntt_pass_contig_group(const PassArgs *jobs, const u32 *wg_start,
                                                                  u32 njobs) {
   const u32 j = group_of(wg_start, njobs, blockIdx.x);
   contig_body<S, N2B, INV, SC, WHOLE>(jobs[j], blockIdx.x - wg_start[j]);
