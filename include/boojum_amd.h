@@ -317,7 +317,8 @@ int ba_server_poll(ba_server_t server, uint64_t ticket, int *done, ba_serve_resu
 /* returns a finished request's outputs to the arena; the ticket is then forgotten */
 int ba_server_release(ba_server_t server, uint64_t ticket);
 int ba_server_stats(ba_server_t server, ba_serve_stats *stats);
-/* completes every submitted request, then frees the arena and the slots */
+/* completes every submitted request, then frees the arena and the slots; no other call on the
+ * same server may run concurrently with it or follow it */
 int ba_server_destroy(ba_server_t server);
 
 /* ---------------------------------------------------------------- launch tracing
