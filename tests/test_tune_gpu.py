@@ -69,6 +69,27 @@ def test_every_split_bit_exact(ba, log_n):
         ba.tune.set_plan(ba._test_ctx, log_n, br, None)
 
 
+@pytest.mark.parametrize("log_n", [13, 16])
+def test_scaled_contiguous_pass_columns_per_workgroup(ba, log_n):
+    """The two modes whose per-element coset factors are fused into the contiguous pass
+    (forward b2n: before its first round; inverse n2b: after its last round) with 1, 2, 4 and 8
+    columns per workgroup: each workgroup forms the factors anew for every column it repeats
+    over, and every column matches the oracle."""
+    cols = 8
+    n = 1 << log_n
+    rng = np.random.default_rng(100 + log_n)
+    x = raw_u64(rng, cols * n)
+    for bitrev, inverse in ((True, False), (False, True)):
+        want = gl_oracle.batch_ntt(x, log_n, cols, n, n, bitrev, inverse, 2, 1, nthreads=8)
+        bits, _ = ba.tune.get_plan(ba._test_ctx, log_n, bitrev)
+        for r in (1, 2, 4, 8):
+            log(f"scaled contiguous pass log_n={log_n} bitrev={bitrev} inverse={inverse} reps={r}")
+            ba.tune.set_plan(ba._test_ctx, log_n, bitrev, bits, [r] * len(bits))
+            got = run_ntt(ba, x, log_n, cols, bitrev, inverse, 2, 1)
+            assert np.array_equal(got, want), (log_n, bitrev, inverse, r)
+        ba.tune.set_plan(ba._test_ctx, log_n, bitrev, None)
+
+
 def test_set_plan_rejects_invalid(ba):
     t = ba.tune
     bad = [
