@@ -487,3 +487,87 @@ void gl_fill_uniform(uint64_t *v, size_t n, uint64_t seed) {
     v[i] = x;
   }
 }
+
+/* ---------------------------------------------------------------- Poseidon2 */
+
+static const uint64_t P2_M4[4][4] = {{5, 7, 1, 3}, {4, 6, 1, 1}, {1, 3, 5, 7}, {1, 1, 4, 6}};
+static const unsigned P2_MU[12] = {4, 14, 11, 8, 0, 5, 2, 9, 13, 6, 3, 12};
+
+/* y = M_E x with M_E's 4 x 4 blocks 2 M4 on the diagonal and M4 elsewhere */
+static void p2_external(uint64_t x[12]) {
+  uint64_t y[12];
+  for (int r = 0; r < 12; r++) {
+    uint64_t acc = 0;
+    for (int c = 0; c < 12; c++) {
+      uint64_t m = P2_M4[r % 4][c % 4];
+      if (r / 4 == c / 4) m *= 2;
+      acc = gl_add(acc, gl_mul(m, x[c]));
+    }
+    y[r] = acc;
+  }
+  memcpy(x, y, sizeof(y));
+}
+
+/* y = (J + diag(2^mu)) x */
+static void p2_internal(uint64_t x[12]) {
+  uint64_t sum = 0;
+  for (int i = 0; i < 12; i++) sum = gl_add(sum, x[i]);
+  for (int i = 0; i < 12; i++) x[i] = gl_add(sum, gl_mul(x[i], 1ull << P2_MU[i]));
+}
+
+static uint64_t p2_sbox(uint64_t x) { return gl_pow(x, 7); }
+
+void gl_oracle_poseidon2_matrix(int internal, uint64_t x[12]) {
+  for (int i = 0; i < 12; i++) x[i] = gl_reduce(x[i]);
+  if (internal)
+    p2_internal(x);
+  else
+    p2_external(x);
+}
+
+void gl_oracle_poseidon2_permutation(uint64_t state[12], const uint64_t *rc) {
+  for (int i = 0; i < 12; i++) state[i] = gl_reduce(state[i]);
+  p2_external(state);
+  for (int r = 0; r < 30; r++) {
+    const int full = r < 4 || r >= 26;
+    if (full) {
+      for (int i = 0; i < 12; i++) state[i] = p2_sbox(gl_add(state[i], gl_reduce(rc[12 * r + i])));
+      p2_external(state);
+    } else {
+      state[0] = p2_sbox(gl_add(state[0], gl_reduce(rc[12 * r])));
+      p2_internal(state);
+    }
+  }
+}
+
+void gl_oracle_poseidon2_leaves(const uint64_t *values, uint64_t *results, uint32_t rows_count,
+                                uint32_t cols_count, uint32_t count, int load_intermediate,
+                                int store_intermediate, const uint64_t *rc) {
+  const uint64_t total = (uint64_t)rows_count * cols_count;
+  for (uint64_t gid = 0; gid < count; gid++) {
+    uint64_t state[12] = {0};
+    if (load_intermediate)
+      for (int k = 0; k < 4; k++) state[8 + k] = results[gid + (uint64_t)k * count];
+    for (uint64_t off = 0; off < total; off += 8) {
+      for (int i = 0; i < 8; i++) {
+        const uint64_t o = off + i, r = o % rows_count, c = o / rows_count;
+        state[i] = c < cols_count ? values[gid * rows_count + r + c * rows_count * (uint64_t)count] : 0;
+      }
+      gl_oracle_poseidon2_permutation(state, rc);
+    }
+    for (int k = 0; k < 4; k++) results[gid + (uint64_t)k * count] = state[(store_intermediate ? 8 : 0) + k];
+  }
+}
+
+void gl_oracle_poseidon2_nodes(const uint64_t *values, uint64_t *results, uint32_t count,
+                               const uint64_t *rc) {
+  for (uint64_t gid = 0; gid < count; gid++) {
+    uint64_t state[12] = {0};
+    for (int k = 0; k < 4; k++) {
+      state[k] = values[2 * (gid + (uint64_t)k * count)];
+      state[4 + k] = values[2 * (gid + (uint64_t)k * count) + 1];
+    }
+    gl_oracle_poseidon2_permutation(state, rc);
+    for (int k = 0; k < 4; k++) results[gid + (uint64_t)k * count] = state[k];
+  }
+}

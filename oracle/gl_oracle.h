@@ -129,6 +129,32 @@ int gl_oracle_lde(const uint64_t *evals, uint64_t in_stride, uint64_t *out,
 /* SplitMix64 with rejection of x >= p: the seeded input generator of the benchmarks */
 void gl_fill_uniform(uint64_t *v, size_t n, uint64_t seed);
 
+/*
+ * Poseidon2 over Goldilocks, width 12 (rate 8, capacity 4): Merkle leaf and node hashes
+ * (crates/boojum-cuda/src/poseidon2.rs:165-300, kernels
+ * crates/boojum-cuda/native/poseidon2/gl/poseidon2_gl_st.cu:24-88).  The permutation follows
+ * the Poseidon2 paper (eprint 2023/323, Fig. 1 and Appendix B) written as matrix products:
+ * M_E = circ(2 M4, M4, M4) with M4 = [[5,7,1,3],[4,6,1,1],[1,3,5,7],[1,1,4,6]], applied once
+ * first; 4 full rounds (constants on all 12 lanes, x^7 on all, M_E), 22 partial rounds
+ * (constant and x^7 on lane 0, M_I = J + diag(2^mu) with mu = {4, 14, 11, 8, 0, 5, 2, 9, 13, 6,
+ * 3, 12}, poseidon2_gl_constants.cuh.template:13), 4 full rounds.  The 30 x 12 round constants
+ * are an argument: boojum's table (poseidon_goldilocks_params) is not in the reference tree.
+ */
+void gl_oracle_poseidon2_permutation(uint64_t state[12], const uint64_t *round_constants);
+/* the linear layers alone: M_E x (internal = 0) or M_I x (internal = 1), in place */
+void gl_oracle_poseidon2_matrix(int internal, uint64_t x[12]);
+/* leaf gid absorbs, 8 at a time in overwrite mode (zero-filled past the end), the values
+ * values[gid * rows + r + c * rows * count] for c < cols, r < rows, column by column; digest
+ * word k goes to results[gid + k * count] (state[k], or the capacity state[8 + k] when
+ * store_intermediate; load_intermediate starts from a stored capacity) */
+void gl_oracle_poseidon2_leaves(const uint64_t *values, uint64_t *results, uint32_t rows_count,
+                                uint32_t cols_count, uint32_t count, int load_intermediate,
+                                int store_intermediate, const uint64_t *round_constants);
+/* node gid hashes the two child digests of the layer below: word k of the left child at
+ * values[2 (gid + k * count)], of the right child at values[2 (gid + k * count) + 1] */
+void gl_oracle_poseidon2_nodes(const uint64_t *values, uint64_t *results, uint32_t count,
+                               const uint64_t *round_constants);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,10 @@ def lib():
             ("gl_oracle_omega_shift", None, [_u64p, u32, u32, i32, u32, _u64p, u32]),
             ("gl_oracle_barycentric_coeffs", None, [i32, u64, u64, u64, u32, i32, _u64p]),
             ("gl_oracle_barycentric_eval", None, [i32, _u64p, u32, u32, _u64p, _u64p]),
+            ("gl_oracle_poseidon2_permutation", None, [_u64p, _u64p]),
+            ("gl_oracle_poseidon2_matrix", None, [i32, _u64p]),
+            ("gl_oracle_poseidon2_leaves", None, [_u64p, _u64p, u32, u32, u32, i32, i32, _u64p]),
+            ("gl_oracle_poseidon2_nodes", None, [_u64p, _u64p, u32, _u64p]),
         ]:
             f = getattr(L, name)
             f.restype = res
@@ -174,3 +178,52 @@ def barycentric_eval(kind, ys, count, num_polys, coeffs):
     r = np.empty(num_polys * (1 if kind == 0 else 2), dtype=np.uint64)
     lib().gl_oracle_barycentric_eval(kind, _ptr(ys), count, num_polys, _ptr(coeffs), _ptr(r))
     return r
+
+
+# ---- Poseidon2 (width 12) Merkle hashing; round_constants: 360 u64 (30 rounds x 12 lanes) ----
+def poseidon2_permutation(state, round_constants):
+    st = np.ascontiguousarray(np.array(state, dtype=np.uint64)).copy()
+    assert st.size == 12
+    lib().gl_oracle_poseidon2_permutation(_ptr(st), _ptr(np.ascontiguousarray(round_constants, dtype=np.uint64)))
+    return st
+
+
+def poseidon2_matrix(internal, x):
+    v = np.ascontiguousarray(np.array(x, dtype=np.uint64)).copy()
+    lib().gl_oracle_poseidon2_matrix(int(internal), _ptr(v))
+    return v
+
+
+def poseidon2_leaves(values, rows_count, cols_count, count, round_constants, load_intermediate=False,
+                     store_intermediate=False, results=None):
+    values = np.ascontiguousarray(values, dtype=np.uint64)
+    out = np.zeros(4 * count, dtype=np.uint64) if results is None else np.ascontiguousarray(results, dtype=np.uint64).copy()
+    lib().gl_oracle_poseidon2_leaves(_ptr(values), _ptr(out), rows_count, cols_count, count, int(load_intermediate),
+                                     int(store_intermediate), _ptr(np.ascontiguousarray(round_constants, dtype=np.uint64)))
+    return out
+
+
+def poseidon2_nodes(values, count, round_constants):
+    values = np.ascontiguousarray(values, dtype=np.uint64)
+    out = np.zeros(4 * count, dtype=np.uint64)
+    lib().gl_oracle_poseidon2_nodes(_ptr(values), _ptr(out), count,
+                                    _ptr(np.ascontiguousarray(round_constants, dtype=np.uint64)))
+    return out
+
+
+def poseidon2_merkle_tree(values, log_rows_per_hash, leaves_count, layers_count, round_constants):
+    """build_merkle_tree's layout (crates/boojum-cuda/src/poseidon2.rs:253-316): 8 x leaves_count
+    words; the leaf digests (4 planes of leaves_count) first, then each node layer (4 planes of
+    half the previous count) after the previous one; the words past the last layer are zero."""
+    out = np.zeros(8 * leaves_count, dtype=np.uint64)
+    rows = 1 << log_rows_per_hash
+    cols = values.size // (leaves_count * rows)
+    layer = poseidon2_leaves(values, rows, cols, leaves_count, round_constants)
+    out[:layer.size] = layer
+    at, count = layer.size, leaves_count
+    for _ in range(layers_count - 1):
+        count //= 2
+        layer = poseidon2_nodes(layer, count, round_constants)
+        out[at:at + layer.size] = layer
+        at += layer.size
+    return out

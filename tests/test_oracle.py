@@ -16,6 +16,7 @@ import pytest
 
 import bigint_ref as br
 import gl_oracle
+from gpu_util import raw_u64
 
 P = br.P
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ntt_golden.npz")
@@ -336,3 +337,124 @@ def test_barycentric_on_lde_cosets():
         shift = 7 * pow(W30, br.bitrev(c, lb) << (30 - log_n - lb), P) % P
         coeffs = gl_oracle.barycentric_coeffs(x, shift, n, True)
         assert int(gl_oracle.barycentric_eval(0, lde[c * n:(c + 1) * n], n, 1, coeffs)[0]) == want, c
+
+
+# ---- Poseidon2 (width 12) Merkle hashing, crates/boojum-cuda/native/poseidon2/gl/ ----
+P2_MU = [4, 14, 11, 8, 0, 5, 2, 9, 13, 6, 3, 12]
+
+
+def _m4_chain(x):
+    """M4 as the reference evaluates it, an addition chain (poseidon2_gl.cuh:9-25), here on
+    Python integers: the matrix it computes must be the paper's M4 (eprint 2023/323, App. B)."""
+    t0 = x[0] + x[1]
+    t1 = x[2] + x[3]
+    t2 = 2 * x[1] + t1
+    t3 = 2 * x[3] + t0
+    t4 = 4 * t1 + t3
+    t5 = 4 * t0 + t2
+    return [t3 + t5, t5, t2 + t4, t4]
+
+
+def test_poseidon2_linear_layers():
+    m4 = [[5, 7, 1, 3], [4, 6, 1, 1], [1, 3, 5, 7], [1, 1, 4, 6]]
+    for j in range(4):  # the chain's columns are the paper's matrix
+        e = [int(i == j) for i in range(4)]
+        assert _m4_chain(e) == [m4[r][j] for r in range(4)]
+    # M_E = (I_3 + J_3) (x) M4, M_I = J_12 + diag(2^mu), as integer matrices mod p
+    me = [[(1 + (r // 4 == c // 4)) * m4[r % 4][c % 4] for c in range(12)] for r in range(12)]
+    mi = [[1 + (r == c) * (1 << P2_MU[r]) for c in range(12)] for r in range(12)]
+    rng = np.random.default_rng(21)
+    for trial in range(40):
+        x = [int(v) for v in rng.integers(0, P, 12, dtype=np.uint64)]
+        if trial == 0:
+            x = [P - 1] * 12
+        for internal, m in ((0, me), (1, mi)):
+            want = [sum(m[r][c] * x[c] for c in range(12)) % P for r in range(12)]
+            assert [int(v) for v in gl_oracle.poseidon2_matrix(internal, x)] == want, (internal, trial)
+    # the reference's M_E formulation: M4 per tile, then each lane plus the sum over tiles
+    x = [int(v) for v in rng.integers(0, P, 12, dtype=np.uint64)]
+    y = sum((_m4_chain(x[4 * t:4 * t + 4]) for t in range(3)), [])
+    acc = [sum(y[4 * t + j] for t in range(3)) for j in range(4)]
+    assert [int(v) for v in gl_oracle.poseidon2_matrix(0, x)] == [(acc[i % 4] + y[i]) % P for i in range(12)]
+
+
+def test_poseidon2_permutation_rounds():
+    """x^7 is a permutation of F_p (gcd(7, p - 1) = 1); the permutation equals the paper's round
+    sequence evaluated on Python integers with the matrices above."""
+    from math import gcd
+
+    assert gcd(7, P - 1) == 1
+    m4 = [[5, 7, 1, 3], [4, 6, 1, 1], [1, 3, 5, 7], [1, 1, 4, 6]]
+    me = [[(1 + (r // 4 == c // 4)) * m4[r % 4][c % 4] for c in range(12)] for r in range(12)]
+    mi = [[1 + (r == c) * (1 << P2_MU[r]) for c in range(12)] for r in range(12)]
+
+    def mat(m, x):
+        return [sum(m[r][c] * x[c] for c in range(12)) % P for r in range(12)]
+
+    rng = np.random.default_rng(22)
+    for trial in range(4):
+        rc = rng.integers(0, P, 360, dtype=np.uint64)
+        x = [int(v) for v in rng.integers(0, P, 12, dtype=np.uint64)]
+        got = gl_oracle.poseidon2_permutation(x, rc)
+        s = mat(me, x)
+        for r in range(30):
+            if r < 4 or r >= 26:
+                s = mat(me, [pow((s[i] + int(rc[12 * r + i])) % P, 7, P) for i in range(12)])
+            else:
+                s = mat(mi, [pow((s[0] + int(rc[12 * r])) % P, 7, P)] + s[1:])
+        assert [int(v) for v in got] == s, trial
+
+
+def test_poseidon2_sponge_layouts():
+    """Leaves, nodes and the tree against Python loops over the oracle's permutation: leaf gid
+    reads values[gid * rows + r + c * rows * count] column by column (the reference's CPU check,
+    crates/boojum-cuda/src/poseidon2.rs:575-590), 8 per permutation in overwrite mode, zero-filled;
+    a node hashes left || right; intermediate capacities chain partial column sets."""
+    rng = np.random.default_rng(23)
+    rc = rng.integers(0, P, 360, dtype=np.uint64)
+    for rows, cols, count in ((1, 5, 6), (2, 7, 4), (4, 4, 3), (8, 3, 2)):
+        values = raw_u64(rng, rows * cols * count)
+        got = gl_oracle.poseidon2_leaves(values, rows, cols, count, rc)
+        for gid in range(count):
+            seq = [int(values[gid * rows + o % rows + (o // rows) * rows * count]) % P for o in range(rows * cols)]
+            st = np.zeros(12, dtype=np.uint64)
+            for off in range(0, len(seq), 8):
+                chunk = seq[off:off + 8] + [0] * max(0, off + 8 - len(seq))
+                st[:8] = chunk
+                st = gl_oracle.poseidon2_permutation(st, rc)
+            assert [int(got[gid + k * count]) for k in range(4)] == [int(v) for v in st[:4]], (rows, cols, gid)
+    # two calls over column halves (intermediate capacity) equal one call over all columns, when
+    # the first half fills whole permutations
+    rows, count = 2, 5
+    values = raw_u64(rng, rows * 8 * count)
+    whole = gl_oracle.poseidon2_leaves(values, rows, 8, count, rc)
+    half = rows * 4 * count
+    mid = gl_oracle.poseidon2_leaves(values[:half], rows, 4, count, rc, store_intermediate=True)
+    done = gl_oracle.poseidon2_leaves(values[half:], rows, 4, count, rc, load_intermediate=True, results=mid)
+    assert np.array_equal(done, whole)
+    # nodes: word k of the left child at values[2 (gid + k count)], the right child next to it
+    count = 4
+    children = raw_u64(rng, 8 * count) % np.uint64(P)
+    got = gl_oracle.poseidon2_nodes(children, count, rc)
+    for gid in range(count):
+        st = np.zeros(12, dtype=np.uint64)
+        for k in range(4):
+            st[k] = children[2 * (gid + k * count)]
+            st[4 + k] = children[2 * (gid + k * count) + 1]
+        assert np.array_equal(got[gid::count][:4], gl_oracle.poseidon2_permutation(st, rc)[:4])
+    # tree: leaf digests, then each node layer hashes adjacent pairs of the layer below
+    leaves, layers = 8, 4
+    values = raw_u64(rng, leaves * 2 * 3)
+    tree = gl_oracle.poseidon2_merkle_tree(values, 1, leaves, layers, rc)
+    layer = [[int(tree[g + k * leaves]) for k in range(4)] for g in range(leaves)]
+    at, n = 4 * leaves, leaves
+    for _ in range(layers - 1):
+        n //= 2
+        nxt = []
+        for g in range(n):
+            st = np.array(layer[2 * g] + layer[2 * g + 1] + [0] * 4, dtype=np.uint64)
+            nxt.append([int(v) for v in gl_oracle.poseidon2_permutation(st, rc)[:4]])
+        assert [[int(tree[at + g + k * n]) for k in range(4)] for g in range(n)] == nxt
+        at += 4 * n
+        layer = nxt
+    assert not tree[at:].any()
