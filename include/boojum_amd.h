@@ -321,6 +321,38 @@ int ba_server_stats(ba_server_t server, ba_serve_stats *stats);
  * same server may run concurrently with it or follow it */
 int ba_server_destroy(ba_server_t server);
 
+/* ---------------------------------------------------------------- Poseidon2 Merkle hashing
+ * boojum_cuda::poseidon2::GpuTreeHasher for the Goldilocks Poseidon2 hasher
+ * (crates/boojum-cuda/src/poseidon2.rs:165-316; kernels
+ * crates/boojum-cuda/native/poseidon2/gl/poseidon2_gl_st.cu:24-88): width 12, rate 8, capacity
+ * 4, the paper's permutation (eprint 2023/323) with 4 + 22 + 4 rounds.  Slices are (pointer,
+ * length in u64 words) with the Rust functions' lengths, layouts and preconditions (a violated
+ * assert! returns BA_ERROR_INVALID_VALUE before anything is launched).  Digests are 4 planes:
+ * word k of digest g at results[g + k * count].  The calls use the current device's context and
+ * return BA_ERROR_NOT_INITIALIZED until its round constants are installed. */
+/* installs boojum's 30 x 12 round constants (poseidon_goldilocks_params
+ * ALL_ROUND_CONSTANTS_AS_FIELD_ELEMENTS, row-major by round; read mod p) into ctx; no reference
+ * counterpart: the reference compiles them into its kernels (build/poseidon2_gl.rs) */
+int ba_poseidon2_gl_set_round_constants(ba_context_t ctx, const uint64_t *round_constants);
+/* GpuTreeHasher::compute_leaf_hashes (poseidon2.rs:180-229) */
+int ba_poseidon2_gl_compute_leaf_hashes(const uint64_t *values, size_t values_len, uint64_t *results,
+                                        size_t results_len, uint32_t log_rows_per_hash, int load_intermediate,
+                                        int store_intermediate, ba_stream_t stream);
+/* GpuTreeHasher::compute_node_hashes (poseidon2.rs:231-251) */
+int ba_poseidon2_gl_compute_node_hashes(const uint64_t *values, size_t values_len, uint64_t *results,
+                                        size_t results_len, ba_stream_t stream);
+/* GpuTreeHasher::build_merkle_tree_leaves (poseidon2.rs:253-274) */
+int ba_poseidon2_gl_build_merkle_tree_leaves(const uint64_t *values, size_t values_len, uint64_t *results,
+                                             size_t results_len, uint32_t log_rows_per_hash, int load_intermediate,
+                                             int store_intermediate, ba_stream_t stream);
+/* GpuTreeHasher::build_merkle_tree_nodes (poseidon2.rs:276-295) */
+int ba_poseidon2_gl_build_merkle_tree_nodes(const uint64_t *values, size_t values_len, uint64_t *results,
+                                            size_t results_len, uint32_t layers_count, ba_stream_t stream);
+/* GpuTreeHasher::build_merkle_tree (poseidon2.rs:297-316) */
+int ba_poseidon2_gl_build_merkle_tree(const uint64_t *values, size_t values_len, uint64_t *results,
+                                      size_t results_len, uint32_t log_rows_per_hash, uint32_t layers_count,
+                                      ba_stream_t stream);
+
 /* ---------------------------------------------------------------- launch tracing
  * No reference counterpart beyond its nvtx ranges (crates/shivini/Cargo.toml:31): after
  * ba_profile_begin(events, capacity) on a host thread, each kernel the library launches from

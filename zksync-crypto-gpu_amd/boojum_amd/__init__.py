@@ -14,4 +14,4 @@ from ._lib import BoojumAmdError, check, lib, library_path  # noqa: F401
 from .runtime import DeviceAllocation, DeviceSlice, CudaStream, CudaEvent  # noqa: F401
 from .runtime import device_count, synchronize, mem_get_info, set_device  # noqa: F401
 from .context import Context, OMEGA_LOG_ORDER  # noqa: F401
-from . import barycentric, batch, lde, ntt, ops, profile, serve, tune  # noqa: F401
+from . import barycentric, batch, lde, ntt, ops, poseidon2, profile, serve, tune  # noqa: F401

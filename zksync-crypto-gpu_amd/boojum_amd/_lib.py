@@ -101,6 +101,12 @@ SIGNATURES = {
     "ba_barycentric_precompute_lagrange_coeffs": [_i32, _vp, _vp, _u64, _vp, _u32, _i32, _vp],
     "ba_barycentric_batch_eval_temp_elems": [_i32, _u32, _u32, ctypes.POINTER(_u64)],
     "ba_barycentric_batch_eval": [_i32, _vp, _u32, _u32, _vp, _vp, _u64, _vp, _vp],
+    "ba_poseidon2_gl_set_round_constants": [_vp, _vp],
+    "ba_poseidon2_gl_compute_leaf_hashes": [_vp, _sz, _vp, _sz, _u32, _i32, _i32, _vp],
+    "ba_poseidon2_gl_compute_node_hashes": [_vp, _sz, _vp, _sz, _vp],
+    "ba_poseidon2_gl_build_merkle_tree_leaves": [_vp, _sz, _vp, _sz, _u32, _i32, _i32, _vp],
+    "ba_poseidon2_gl_build_merkle_tree_nodes": [_vp, _sz, _vp, _sz, _u32, _vp],
+    "ba_poseidon2_gl_build_merkle_tree": [_vp, _sz, _vp, _sz, _u32, _u32, _vp],
     "ba_server_create": [_vp, _sz, _u32, ctypes.POINTER(_vp)],
     "ba_server_submit": [_vp, ctypes.POINTER(ServeRequest), ctypes.POINTER(_u64)],
     "ba_server_wait": [_vp, _u64, ctypes.POINTER(ServeResult)],

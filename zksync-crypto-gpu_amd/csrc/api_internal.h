@@ -34,6 +34,8 @@ struct ba_context {
   BaTuned tuned[2][BA_MAX_LOG_N + 1] = {};  // [n2b][log_n]
   // batch plans and servers hold pointers into the tables: the context outlives them
   std::atomic<int> users{0};
+  // Poseidon2 round constants (30 x 12, device memory), installed by the caller
+  uint64_t *p2_round_constants = nullptr;
 };
 
 // The most recently created live context of `device`, or null.

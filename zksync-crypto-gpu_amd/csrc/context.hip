@@ -127,7 +127,11 @@ extern "C" int ba_context_destroy(ba_context_t ctx) {
       }
     if (!found) return BA_ERROR_INVALID_VALUE;
   }
-  const hipError_t err = hipFree(ctx->mem);
+  hipError_t err = hipFree(ctx->mem);
+  if (ctx->p2_round_constants) {
+    const hipError_t e2 = hipFree(ctx->p2_round_constants);
+    if (err == hipSuccess) err = e2;
+  }
   delete ctx;
   return (int)err;
 }

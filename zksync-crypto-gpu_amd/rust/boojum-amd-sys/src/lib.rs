@@ -210,6 +210,26 @@ extern "C" {
     pub fn ba_server_stats(server: ba_server_t, stats: *mut ba_serve_stats) -> c_int;
     pub fn ba_server_destroy(server: ba_server_t) -> c_int;
 
+    // Poseidon2 Merkle hashing (crates/boojum-cuda/src/poseidon2.rs:165-316): slices as
+    // (pointer, length in u64 words); the context's round constants must be installed first
+    pub fn ba_poseidon2_gl_set_round_constants(ctx: ba_context_t, round_constants: *const u64) -> c_int;
+    pub fn ba_poseidon2_gl_compute_leaf_hashes(values: *const u64, values_len: usize, results: *mut u64,
+                                               results_len: usize, log_rows_per_hash: u32,
+                                               load_intermediate: c_int, store_intermediate: c_int,
+                                               stream: ba_stream_t) -> c_int;
+    pub fn ba_poseidon2_gl_compute_node_hashes(values: *const u64, values_len: usize, results: *mut u64,
+                                               results_len: usize, stream: ba_stream_t) -> c_int;
+    pub fn ba_poseidon2_gl_build_merkle_tree_leaves(values: *const u64, values_len: usize, results: *mut u64,
+                                                    results_len: usize, log_rows_per_hash: u32,
+                                                    load_intermediate: c_int, store_intermediate: c_int,
+                                                    stream: ba_stream_t) -> c_int;
+    pub fn ba_poseidon2_gl_build_merkle_tree_nodes(values: *const u64, values_len: usize, results: *mut u64,
+                                                   results_len: usize, layers_count: u32,
+                                                   stream: ba_stream_t) -> c_int;
+    pub fn ba_poseidon2_gl_build_merkle_tree(values: *const u64, values_len: usize, results: *mut u64,
+                                             results_len: usize, log_rows_per_hash: u32, layers_count: u32,
+                                             stream: ba_stream_t) -> c_int;
+
     // launch tracing and helpers
     pub fn ba_profile_begin(events: *mut ba_event_t, capacity: u32) -> c_int;
     pub fn ba_profile_end(launches: *mut u32) -> c_int;
