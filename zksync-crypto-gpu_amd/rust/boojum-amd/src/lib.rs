@@ -2,8 +2,8 @@
 //!
 //! Module and function names follow the reference crates so a caller switches by changing
 //! its `use` lines: `runtime` stands for era_cudart (crates/cudart/src/{stream,event,memory,
-//! slice}), `context`, `ntt`, `ops_complex` and `barycentric` for boojum_cuda's modules of the
-//! same names.  Field elements are `u64` (boojum's `GoldilocksField` is `#[repr(transparent)]`
+//! slice}), `context`, `ntt`, `ops_complex`, `barycentric` and `poseidon2` for boojum_cuda's
+//! modules of the same names.  Field elements are `u64` (boojum's `GoldilocksField` is `#[repr(transparent)]`
 //! over `u64`, so a caller's `DeviceSlice<GoldilocksField>` is viewed as `DeviceSlice<u64>`).
 //! Status 1 from the library is where the reference asserts: these wrappers keep that panic,
 //! and return every other error as `Err`.
@@ -12,6 +12,7 @@ pub mod batch;
 pub mod context;
 pub mod ntt;
 pub mod ops_complex;
+pub mod poseidon2;
 pub mod runtime;
 pub mod serve;
 
