@@ -378,6 +378,14 @@ int ba_fill_uniform(uint64_t *dst, uint64_t stride, uint64_t rows, uint32_t cols
 int ba_field_op(int op, const uint64_t *a, const uint64_t *b, uint64_t *out, size_t n,
                 ba_stream_t stream);
 
+/* Diagnostic, no reference counterpart: blocks x 256 threads each run `iters` x 8 butterflies
+ * (one general product, one canonical add and subtract each, the NTT passes' arithmetic) on 8
+ * register values seeded from in[thread], writing one word to out[thread] (in and out hold
+ * blocks x 256 words).  Timing it gives the VALU issue rate of the passes' instruction mix
+ * without memory, LDS or barriers. */
+int ba_diag_butterfly_rate(const uint64_t *in, uint64_t *out, uint32_t blocks, uint32_t iters,
+                           ba_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

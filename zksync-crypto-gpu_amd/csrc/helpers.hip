@@ -47,7 +47,38 @@ __global__ __launch_bounds__(256) void field_op_kernel(int op, const u64 *a, con
   }
 }
 
+// Register-only loop over the NTT passes' butterfly arithmetic (one general product, one
+// canonical add and one canonical subtract per butterfly, 8 independent values per thread): the
+// VALU issue rate this instruction mix reaches with no memory, LDS or barrier in the way.
+constexpr int kBflyValues = 8;
+__global__ __launch_bounds__(256) void butterfly_rate_kernel(const u64 *in, u64 *out, u32 iters) {
+  const u64 i = blockIdx.x * 256ull + threadIdx.x;
+  const u64 w = gl::canon(in[i]) | 1;
+  u64 v[kBflyValues];
+#pragma unroll
+  for (int c = 0; c < kBflyValues; c++) v[c] = gl::canon(in[i] + c);
+  for (u32 it = 0; it < iters; it++) {
+#pragma unroll
+    for (int c = 0; c < kBflyValues; c++) {
+      const u64 a = v[c], t = gl::mul(v[(c + 1) % kBflyValues], w);
+      v[c] = gl::add(a, t);
+      v[(c + 1) % kBflyValues] = gl::sub(a, t);
+    }
+  }
+  u64 acc = 0;
+#pragma unroll
+  for (int c = 0; c < kBflyValues; c++) acc ^= v[c];
+  out[i] = acc;
+}
+
 }  // namespace
+
+extern "C" int ba_diag_butterfly_rate(const uint64_t *in, uint64_t *out, uint32_t blocks, uint32_t iters,
+                                      ba_stream_t stream) {
+  if (!in || !out || blocks == 0) return BA_ERROR_INVALID_VALUE;
+  hipLaunchKernelGGL(butterfly_rate_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, in, out, iters);
+  return (int)hipGetLastError();
+}
 
 extern "C" int ba_fill_uniform(uint64_t *dst, uint64_t stride, uint64_t rows, uint32_t cols,
                                uint64_t seed, ba_stream_t stream) {

@@ -210,6 +210,10 @@ extern "C" {
     pub fn ba_server_stats(server: ba_server_t, stats: *mut ba_serve_stats) -> c_int;
     pub fn ba_server_destroy(server: ba_server_t) -> c_int;
 
+    // diagnostic: the VALU rate of the passes' butterfly arithmetic in registers
+    pub fn ba_diag_butterfly_rate(input: *const u64, output: *mut u64, blocks: u32, iters: u32,
+                                  stream: ba_stream_t) -> c_int;
+
     // Poseidon2 Merkle hashing (crates/boojum-cuda/src/poseidon2.rs:165-316): slices as
     // (pointer, length in u64 words); the context's round constants must be installed first
     pub fn ba_poseidon2_gl_set_round_constants(ctx: ba_context_t, round_constants: *const u64) -> c_int;
