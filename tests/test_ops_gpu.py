@@ -185,3 +185,32 @@ def test_omega_shift(ba, tables):
             ba.ops.get_powers_of_w(31, 0, False, False, a)
     finally:
         a.free()
+
+
+def test_diag_butterfly_rate_output(ba):
+    """ba_diag_butterfly_rate's word per thread equals the same butterfly loop on Python integers
+    (v[c], v[c+1] <- v[c] + w v[c+1], v[c] - w v[c+1], 8 values, xor of the final values)."""
+
+    blocks, iters = 1, 3
+    rng = np.random.default_rng(31)
+    x = raw_u64(rng, 256 * blocks)
+    d_in, d_out = ba.DeviceAllocation.from_host(x), ba.DeviceAllocation(256 * blocks)
+    try:
+        ba.check(ba.lib().ba_diag_butterfly_rate(d_in.ptr, d_out.ptr, blocks, iters, None), "butterfly_rate")
+        ba.synchronize()
+        got = d_out.to_host()
+    finally:
+        d_in.free()
+        d_out.free()
+    for t in range(0, 256, 17):
+        xi = int(x[t])
+        w = (xi % P) | 1
+        v = [((xi + c) % 2**64) % P for c in range(8)]
+        for _ in range(iters):
+            for c in range(8):
+                a, b = v[c], v[(c + 1) % 8] * w % P
+                v[c], v[(c + 1) % 8] = (a + b) % P, (a - b) % P
+        acc = 0
+        for c in range(8):
+            acc ^= v[c]
+        assert int(got[t]) == acc, t
