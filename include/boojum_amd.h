@@ -332,7 +332,8 @@ int ba_server_destroy(ba_server_t server);
  * return BA_ERROR_NOT_INITIALIZED until its round constants are installed. */
 /* installs boojum's 30 x 12 round constants (poseidon_goldilocks_params
  * ALL_ROUND_CONSTANTS_AS_FIELD_ELEMENTS, row-major by round; read mod p) into ctx; no reference
- * counterpart: the reference compiles them into its kernels (build/poseidon2_gl.rs) */
+ * counterpart: the reference compiles them into its kernels (build/poseidon2_gl.rs).  The copy is
+ * synchronous; replacing constants while hashing work on the context is in flight races with it */
 int ba_poseidon2_gl_set_round_constants(ba_context_t ctx, const uint64_t *round_constants);
 /* GpuTreeHasher::compute_leaf_hashes (poseidon2.rs:180-229) */
 int ba_poseidon2_gl_compute_leaf_hashes(const uint64_t *values, size_t values_len, uint64_t *results,
