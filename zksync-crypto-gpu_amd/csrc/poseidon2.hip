@@ -12,8 +12,9 @@
 // round constants come from boojum (poseidon_goldilocks_params), which the reference tree does
 // not hold: a caller installs them once per context with ba_poseidon2_gl_set_round_constants.
 //
-// One thread per digest holds the 12-lane state in registers with canonical arithmetic; the
-// round constants are read with wave-uniform indices from the context's copy.
+// One thread per digest holds the 12-lane state in registers (canonical between layers; the
+// linear layers sum exactly and reduce once per output); the round constants are read with
+// wave-uniform indices from the context's copy.
 #include <vector>
 
 #include "api_internal.h"
@@ -28,59 +29,80 @@ constexpr int WIDTH = 12, RATE = 8, CAPACITY = 4;
 constexpr int HALF_FULL_ROUNDS = 4, PARTIAL_ROUNDS = 22, ROUNDS = 2 * HALF_FULL_ROUNDS + PARTIAL_ROUNDS;
 constexpr int THREADS = 256;
 
-__device__ __forceinline__ u64 twice(u64 x) { return gl::add(x, x); }
+// The linear layers are evaluated on exact integers held as (lo, hi) words and reduced once per
+// output (gl::reduce2 takes any lo + hi 2^64 with hi < 2^32): with canonical inputs M4's outputs
+// are < 16 p, M_E's < 64 p and M_I's < (12 + 2^14) p, far inside that range, and each output is
+// the same residue a reduction after every addition would give.
+struct Wide {
+  u64 lo;
+  u32 hi;
+};
+__device__ __forceinline__ Wide wide(u64 x) { return {x, 0}; }
+__device__ __forceinline__ Wide operator+(Wide a, Wide b) {
+  u64 lo;
+  const bool c = __builtin_add_overflow(a.lo, b.lo, &lo);
+  return {lo, a.hi + b.hi + (u32)c};
+}
+template <int K>
+__device__ __forceinline__ Wide shifted(Wide a) {  // a 2^K, K < 32
+  return {a.lo << K, (a.hi << K) | (u32)(a.lo >> (64 - K))};
+}
+__device__ __forceinline__ u64 reduced(Wide a) { return gl::reduce2(a.lo, a.hi); }
 
 // M4 of Appendix B, [[5,7,1,3],[4,6,1,1],[1,3,5,7],[1,1,4,6]], as an addition chain
-__device__ __forceinline__ void m4(u64 &x0, u64 &x1, u64 &x2, u64 &x3) {
-  const u64 a = gl::add(x0, x1), b = gl::add(x2, x3);
-  const u64 c = gl::add(twice(x1), b);            // 2 x1 + x2 + x3
-  const u64 d = gl::add(twice(x3), a);            // x0 + x1 + 2 x3
-  const u64 e = gl::add(twice(twice(b)), d);      // x0 + x1 + 4 x2 + 6 x3
-  const u64 f = gl::add(twice(twice(a)), c);      // 4 x0 + 6 x1 + x2 + x3
-  x0 = gl::add(d, f);
+__device__ __forceinline__ void m4(Wide &x0, Wide &x1, Wide &x2, Wide &x3) {
+  const Wide a = x0 + x1, b = x2 + x3;
+  const Wide c = shifted<1>(x1) + b;  // 2 x1 + x2 + x3
+  const Wide d = shifted<1>(x3) + a;  // x0 + x1 + 2 x3
+  const Wide e = shifted<2>(b) + d;   // x0 + x1 + 4 x2 + 6 x3
+  const Wide f = shifted<2>(a) + c;   // 4 x0 + 6 x1 + x2 + x3
+  x0 = d + f;
   x1 = f;
-  x2 = gl::add(c, e);
+  x2 = c + e;
   x3 = e;
 }
 
 // M_E: M4 on each 4-lane tile, then every lane plus the sum of its position over the tiles
 __device__ __forceinline__ void external_layer(u64 (&s)[WIDTH]) {
+  Wide w[WIDTH];
 #pragma unroll
-  for (int t = 0; t < WIDTH; t += 4) m4(s[t], s[t + 1], s[t + 2], s[t + 3]);
+  for (int i = 0; i < WIDTH; i++) w[i] = wide(s[i]);
+#pragma unroll
+  for (int t = 0; t < WIDTH; t += 4) m4(w[t], w[t + 1], w[t + 2], w[t + 3]);
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    const u64 sum = gl::add(gl::add(s[j], s[4 + j]), s[8 + j]);
-    s[j] = gl::add(s[j], sum);
-    s[4 + j] = gl::add(s[4 + j], sum);
-    s[8 + j] = gl::add(s[8 + j], sum);
+    const Wide sum = w[j] + w[4 + j] + w[8 + j];
+    s[j] = reduced(w[j] + sum);
+    s[4 + j] = reduced(w[4 + j] + sum);
+    s[8 + j] = reduced(w[8 + j] + sum);
   }
 }
 
 template <int MU>
-__device__ __forceinline__ u64 times_pow2(u64 x) {
+__device__ __forceinline__ u64 sum_plus(Wide sum, u64 x) {  // sum + x 2^MU, reduced
   if constexpr (MU == 0)
-    return x;
+    return reduced(sum + wide(x));
   else
-    return gl::shl<MU>(x);
+    return reduced(sum + Wide{x << MU, (u32)(x >> (64 - MU))});
 }
 
 // M_I = J + diag(2^mu), mu = {4, 14, 11, 8, 0, 5, 2, 9, 13, 6, 3, 12}
 __device__ __forceinline__ void internal_layer(u64 (&s)[WIDTH]) {
-  u64 sum = s[0];
+  Wide sum = wide(s[0]);
 #pragma unroll
-  for (int i = 1; i < WIDTH; i++) sum = gl::add(sum, s[i]);
-  s[0] = gl::add(sum, times_pow2<4>(s[0]));
-  s[1] = gl::add(sum, times_pow2<14>(s[1]));
-  s[2] = gl::add(sum, times_pow2<11>(s[2]));
-  s[3] = gl::add(sum, times_pow2<8>(s[3]));
-  s[4] = gl::add(sum, times_pow2<0>(s[4]));
-  s[5] = gl::add(sum, times_pow2<5>(s[5]));
-  s[6] = gl::add(sum, times_pow2<2>(s[6]));
-  s[7] = gl::add(sum, times_pow2<9>(s[7]));
-  s[8] = gl::add(sum, times_pow2<13>(s[8]));
-  s[9] = gl::add(sum, times_pow2<6>(s[9]));
-  s[10] = gl::add(sum, times_pow2<3>(s[10]));
-  s[11] = gl::add(sum, times_pow2<12>(s[11]));
+  for (int i = 1; i < WIDTH; i++) sum = sum + wide(s[i]);
+  s[0] = sum_plus<4>(sum, s[0]);
+  s[1] = sum_plus<14>(sum, s[1]);
+  s[2] = sum_plus<11>(sum, s[2]);
+  s[3] = sum_plus<8>(sum, s[3]);
+  s[4] = sum_plus<0>(sum, s[4]);
+  s[5] = sum_plus<5>(sum, s[5]);
+  s[6] = sum_plus<2>(sum, s[6]);
+  s[7] = sum_plus<9>(sum, s[7]);
+  s[8] = sum_plus<13>(sum, s[8]);
+  s[9] = sum_plus<6>(sum, s[9]);
+  s[10] = sum_plus<3>(sum, s[10]);
+  s[11] = sum_plus<12>(sum, s[11]);
 }
 
 __device__ __forceinline__ u64 sbox(u64 x) {
@@ -121,7 +143,8 @@ __global__ __launch_bounds__(THREADS) void leaves_kernel(const u64 *__restrict__
   const u64 *leaf = values + (u64)gid * rows;
   const u64 col_stride = (u64)rows * count;
   const u64 total = (u64)rows * cols;
-  u32 r = 0, c = 0;  // (row, column) of the next value, the same for every lane
+  u32 r = 0;  // (row, column) of the next value, the same for every lane
+  u64 c = 0;
   for (u64 off = 0; off < total; off += RATE) {
 #pragma unroll
     for (int i = 0; i < RATE; i++) {
