@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serve-mode leg")
     ap.add_argument("--no-single", action="store_true", help="skip the single 2^16 NTT leg")
     ap.add_argument("--no-merkle", action="store_true", help="skip the Poseidon2 Merkle tree over the LDE output")
+    ap.add_argument("--no-ops", action="store_true", help="skip omega_shift, powers and barycentric evaluation")
     ap.add_argument("--no-harness", action="store_true",
                     help="skip the leg with the reference benchmark's grid (2^28-element matrices)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -304,6 +305,56 @@ def run_merkle(ba, ctx, stream, d_out, log_n, cols, lb, log_rows=1, reps=5):
             "ms": round(ms, 3), "hashed_melem_s": round(hashed / (ms / 1e3) / 1e6, 1),
             "permutations_per_s": round(perms / (ms / 1e3), 1), "permutations": perms,
             "input_GBs": round(8 * hashed / (ms / 1e3) / 1e9, 1)}
+
+
+def run_ops(ba, stream, d_ev, d_tmp, log_n, cols, reps=5):
+    """The table-sharing element-wise kernels and barycentric evaluation at the headline's size
+    (cols x 2^log_n elements, resident in HBM), each against the HBM roofline: omega_shift (read +
+    write, 16 B per element, ops_complex.rs:228-260), get_powers_of_w (write, 8 B per element,
+    ops_complex.rs:31-52) and barycentric batch_eval of every column at one point (read, 8 B per
+    element; the 2^log_n coefficients are re-read per column group, barycentric.rs:197-300)."""
+    import numpy as np
+
+    n, total = 1 << log_n, cols << log_n
+    e0, e1 = ba.CudaEvent(), ba.CudaEvent()
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        stream.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / reps
+
+    out = {}
+    dst = d_tmp.sub(0, total)
+    src = d_ev.sub(0, total)
+    ms = timed(lambda: ba.ops.omega_shift(src, log_n, 0, False, 1, dst, stream))
+    out["omega_shift"] = {"elems": total, "ms": round(ms, 4), "GBs": round(16 * total / ms / 1e6, 1),
+                          "hbm_frac": round(16 * total / ms / 1e6 / HBM_PEAK_GBS, 4)}
+    ms = timed(lambda: ba.ops.get_powers_of_w(26, 0, False, False, dst, stream))
+    out["get_powers_of_w"] = {"elems": total, "ms": round(ms, 4), "GBs": round(8 * total / ms / 1e6, 1),
+                              "hbm_frac": round(8 * total / ms / 1e6 / HBM_PEAK_GBS, 4)}
+    x = ba.DeviceAllocation.from_host(np.array([0x1234567890ABCDEF % 0xFFFFFFFF00000001], dtype=np.uint64))
+    common = ba.DeviceAllocation(1)
+    coeffs = ba.DeviceAllocation(n)
+    evals = ba.DeviceAllocation(cols)
+    temp = ba.DeviceAllocation(max(1, ba.barycentric.batch_eval_temp_elems(ba.barycentric.BASE_AT_BASE, n, cols)))
+    try:
+        ba.barycentric.precompute_lagrange_coeffs(x, common, 7, coeffs, False, stream=stream)
+        ms = timed(lambda: ba.barycentric.batch_eval(ba.barycentric.BASE_AT_BASE, src, n, cols, coeffs, temp, evals,
+                                                     stream))
+        out["barycentric_batch_eval"] = {"elems": total, "ms": round(ms, 4), "GBs": round(8 * total / ms / 1e6, 1),
+                                         "hbm_frac": round(8 * total / ms / 1e6 / HBM_PEAK_GBS, 4)}
+    finally:
+        for d in (x, common, coeffs, evals, temp):
+            d.free()
+        e0.destroy()
+        e1.destroy()
+    return out
 
 
 def run_large(ba, ctx, stream, log_n=22, cols=256, lb=2, steps=3):
@@ -597,6 +648,8 @@ def main():
         line["harness"] = run_harness(ba, stream)
     if not args.no_merkle:
         line["merkle"] = run_merkle(ba, ctx, stream, d_out, log_n, cols, lb)
+    if not args.no_ops:
+        line["ops"] = run_ops(ba, stream, d_ev, d_mono, log_n, cols)
     if not args.no_serve:
         line["serve"] = run_serve(ba, ctx)
     if not args.no_cpu:
