@@ -214,3 +214,37 @@ def test_diag_butterfly_rate_output(ba):
         for c in range(8):
             acc ^= v[c]
         assert int(got[t]) == acc, t
+
+
+def test_stepped_powers_past_the_grid_stride(ba, tables):
+    """More elements than the launch has lanes (2^24 + 4097 > 65536 x 256), so lanes step their
+    power by omega^(stride ...) more than once; offsets and shifts make the u32 exponent wrap.
+    Sampled indices (lane starts, stride boundaries, the end) against big-integer closed forms."""
+    OMEGA = 0x7E9BD009B86A0845  # omega_{2^30}
+    count = (1 << 24) + 4097
+    stride = 65536 * 256
+    idx = sorted({0, 1, 4095, 4096, 4097, stride - 1, stride, stride + 1, stride + 4096, count - 1}
+                 | set(int(v) for v in np.random.default_rng(41).integers(0, count, 300)))
+    d = ba.DeviceAllocation(count)
+    src_host = raw_u64(np.random.default_rng(42), count)
+    d_src = ba.DeviceAllocation.from_host(src_host)
+    try:
+        for log_degree, offset, inverse in ((20, 0xFFFFFC18, False), (24, 12345, True), (30, 7, False)):
+            ba.ops.get_powers_of_w(log_degree, offset, inverse, False, d)
+            ba.synchronize()
+            got = d.to_host()
+            for i in idx:
+                e = ((((i + offset) % 2**32) << (30 - log_degree)) % 2**32) % 2**30
+                want = pow(OMEGA, (-e) % 2**30 if inverse else e, P)
+                assert int(got[i]) == want, (log_degree, offset, inverse, i)
+        for log_degree, offset, inverse, shift in ((24, 0xFFFFF000, False, 3), (20, 5, True, 0x10001)):
+            ba.ops.omega_shift(d_src, log_degree, offset, inverse, shift, d)
+            ba.synchronize()
+            got = d.to_host()
+            for i in idx:
+                e = (((shift * ((i + offset) % 2**32)) % 2**32) << (30 - log_degree)) % 2**32 % 2**30
+                w = pow(OMEGA, (-e) % 2**30 if inverse else e, P)
+                assert int(got[i]) == int(src_host[i]) % P * w % P, (log_degree, offset, inverse, shift, i)
+    finally:
+        d.free()
+        d_src.free()

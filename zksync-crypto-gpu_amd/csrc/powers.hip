@@ -9,7 +9,8 @@
 //   powers_by_val/ref: r[i] = base^((bit_reverse ? brev32(i) : i) + offset), base field or the
 //                quadratic extension u^2 = 7 (its two coordinates stored as two planes of count);
 //   omega_shift: dst[i] = src[i] * omega_{2^30}^(+-((shift * (i + offset)) << (30 - log_degree))).
-// Every kernel is a grid-stride loop, one element per lane per step, coalesced 8-byte accesses.
+// Every kernel is a grid-stride loop, one element per lane per step, coalesced 8-byte accesses;
+// where the exponent is linear in the index a lane steps its power by one multiply per element.
 #include <algorithm>
 
 #include "api_internal.h"
@@ -27,13 +28,28 @@ __device__ __forceinline__ u32 raw_exponent(u32 i, u32 log_degree, bool bit_reve
   return bit_reverse ? gl::ntt::bitrev32(i, log_degree) : i;
 }
 
+// omega_{2^30} has order 2^30 and every u32 wrap below is a multiple of 2^30, so omega^e depends on e
+// mod 2^30 only: an exponent linear in the index (natural order) gives powers a lane can step by
+// one constant factor per grid stride, instead of two dependent table reads per element.
 __global__ __launch_bounds__(kThreads) void powers_of_w_kernel(gl::Tables tab, u32 log_degree, u32 offset,
                                                                bool inverse, bool bit_reverse, u64 *result,
                                                                u32 count) {
   const u32 shift = gl::OMEGA_LOG_ORDER - log_degree;
-  for (u64 i = blockIdx.x * kThreads + threadIdx.x; i < count; i += gridDim.x * kThreads) {
-    const u32 e = raw_exponent((u32)i + offset, log_degree, bit_reverse) << shift;
-    result[i] = gl::pow_w(tab, e, inverse);
+  const u32 stride = gridDim.x * kThreads;
+  u32 i = blockIdx.x * kThreads + threadIdx.x;
+  if (bit_reverse) {
+    for (u64 k = i; k < count; k += stride)
+      result[k] = gl::pow_w(tab, raw_exponent((u32)k + offset, log_degree, true) << shift, inverse);
+    return;
+  }
+  if (i >= count) return;
+  u64 w = gl::pow_w(tab, (i + offset) << shift, inverse);
+  const u64 step = gl::pow_w(tab, stride << shift, inverse);
+  for (;;) {
+    result[i] = w;
+    if (count - i <= stride) break;
+    i += stride;
+    w = gl::mul(w, step);
   }
 }
 
@@ -118,13 +134,22 @@ __global__ __launch_bounds__(kThreads) void powers_ef_kernel(const u64 *base_ptr
   }
 }
 
+// the exponent (shift (i + offset)) << degree_shift is linear in i modulo 2^30: each lane steps its
+// factor by omega^(+-((shift stride) << degree_shift)) per grid stride
 __global__ __launch_bounds__(kThreads) void omega_shift_kernel(gl::Tables tab, const u64 *src, u32 log_degree,
                                                                u32 offset, bool inverse, u32 shift, u64 *dst,
                                                                u32 count) {
   const u32 degree_shift = gl::OMEGA_LOG_ORDER - log_degree;
-  for (u64 i = blockIdx.x * kThreads + threadIdx.x; i < count; i += gridDim.x * kThreads) {
-    const u32 e = (shift * ((u32)i + offset)) << degree_shift;
-    dst[i] = gl::mul(src[i], gl::pow_w(tab, e, inverse));
+  const u32 stride = gridDim.x * kThreads;
+  u32 i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= count) return;
+  u64 w = gl::pow_w(tab, (shift * (i + offset)) << degree_shift, inverse);
+  const u64 step = gl::pow_w(tab, (shift * stride) << degree_shift, inverse);
+  for (;;) {
+    dst[i] = gl::mul(src[i], w);
+    if (count - i <= stride) break;
+    i += stride;
+    w = gl::mul(w, step);
   }
 }
 
