@@ -165,17 +165,37 @@ __global__ __launch_bounds__(kThreads) void partial_kernel(const u64 *ys, const 
   }
   __shared__ T red[kThreads / 64];
   const u32 col_end = min(num_polys, (blockIdx.y + 1) * kColsPerGroup);
-  for (u32 col = blockIdx.y * kColsPerGroup; col < col_end; col++) {
+  // base-field values: the next column's are loaded while the current column is accumulated
+  // (rows past the end of a short column read as zero and contribute nothing); extension values
+  // are read in the loop (two registers each would leave one wave per SIMD)
+  u64 nxt[YE ? 1 : kRows];
+  auto fetch = [&](u32 col) {
+    if constexpr (!YE) {
+#pragma unroll
+      for (int k = 0; k < kRows; k++) {
+        const u64 r = r0 + t + (u64)k * kThreads;
+        nxt[k] = r < count ? ys[(u64)col * count + r] : 0;
+      }
+    }
+  };
+  u32 col = blockIdx.y * kColsPerGroup;
+  if (col < col_end) fetch(col);
+  for (; col < col_end; col++) {
     T acc = Fx::zero();
-    for (int k = 0; k < kRows; k++) {
-      const u64 r = r0 + t + (u64)k * kThreads;
-      if (r >= count) break;
-      if constexpr (YE) {
+    if constexpr (YE) {
+      for (int k = 0; k < kRows; k++) {
+        const u64 r = r0 + t + (u64)k * kThreads;
+        if (r >= count) break;
         const Ext y = F<true>::load(ys + (u64)col * 2 * count, r, count);
         acc = Fx::add(acc, Fx::mul(l[k], T(y)));
-      } else {
-        acc = Fx::add(acc, Fx::scale(l[k], gl::canon(ys[(u64)col * count + r])));
       }
+    } else {
+      u64 cur[kRows];
+#pragma unroll
+      for (int k = 0; k < kRows; k++) cur[k] = nxt[k];
+      if (col + 1 < col_end) fetch(col + 1);
+#pragma unroll
+      for (int k = 0; k < kRows; k++) acc = Fx::add(acc, Fx::scale(l[k], gl::canon(cur[k])));
     }
     // wave reduction through shuffles, then across the 4 waves through LDS
     for (int off = 32; off > 0; off >>= 1) {
