@@ -147,7 +147,13 @@ int ba_bit_reverse_in_place(uint64_t *values, uint64_t stride, uint32_t log_rows
  * shivini stores them.  The monomials, in bit-reversed order, go to `monomials` (stride n)
  * when it is non-null, else they replace the evaluations in place (as into_monomials consumes
  * its storage, storage.rs:155).  Preconditions: 1 <= log_n, 1 <= log_blowup,
- * log_n + log_blowup <= 30, strides >= n, out distinct from evals / monomials. */
+ * log_n + log_blowup <= 30, strides >= n, out distinct from evals / monomials, the context's
+ * device current.  Cosets after the first run on auxiliary streams of the context, forked from
+ * `stream` after the inverse NTT and joined back into it before the call returns (a graph
+ * captured from `stream` holds the same fork and join; under ba_profile_begin every launch stays
+ * on `stream`, in order); calls on one context are serialised
+ * while they enqueue, and no call on the context may run in another thread while a stream
+ * capture holding one of its LDEs is still open. */
 int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_stride, uint64_t *monomials,
                  uint64_t *out, uint64_t out_col_stride, uint64_t out_coset_stride, uint32_t log_n,
                  uint32_t num_cols, uint32_t log_blowup, ba_stream_t stream);

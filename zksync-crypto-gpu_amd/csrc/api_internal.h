@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <mutex>
 
 #include "../../include/boojum_amd.h"
 #include "gl_tables.h"
@@ -36,6 +37,13 @@ struct ba_context {
   std::atomic<int> users{0};
   // Poseidon2 round constants (30 x 12, device memory), installed by the caller
   uint64_t *p2_round_constants = nullptr;
+  // ba_lde_batch runs the coset NTTs after the first on these streams (created on first use),
+  // forked from and joined back into the caller's stream; the mutex keeps two callers' fork and
+  // join events from interleaving
+  static constexpr int kLdeAux = 7;
+  hipStream_t lde_aux[kLdeAux] = {};
+  hipEvent_t lde_fork = nullptr, lde_join[kLdeAux] = {};
+  std::mutex lde_mutex;
 };
 
 // The most recently created live context of `device`, or null.
@@ -134,3 +142,5 @@ int ba_launch_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint
 // launches from that thread is bracketed by two of the caller's events.
 void ba_trace_before(hipStream_t stream, const char *kernel);
 void ba_trace_after(hipStream_t stream);
+// launch tracing is on for this host thread (per-launch timings need launches that do not overlap)
+bool ba_trace_active();

@@ -127,6 +127,14 @@ extern "C" int ba_context_destroy(ba_context_t ctx) {
       }
     if (!found) return BA_ERROR_INVALID_VALUE;
   }
+  for (int i = 0; i < ba_context::kLdeAux; i++) {
+    if (ctx->lde_aux[i]) {
+      (void)hipStreamSynchronize(ctx->lde_aux[i]);
+      (void)hipStreamDestroy(ctx->lde_aux[i]);
+    }
+    if (ctx->lde_join[i]) (void)hipEventDestroy(ctx->lde_join[i]);
+  }
+  if (ctx->lde_fork) (void)hipEventDestroy(ctx->lde_fork);
   hipError_t err = hipFree(ctx->mem);
   if (ctx->p2_round_constants) {
     const hipError_t e2 = hipFree(ctx->p2_round_constants);

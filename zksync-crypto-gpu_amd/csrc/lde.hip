@@ -96,11 +96,47 @@ extern "C" int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_strid
   int rc = ba_validate_lde(evals, in_stride, monomials, out, out_col_stride, out_coset_stride, log_n,
                            num_cols, log_blowup);
   if (rc || num_cols == 0) return rc;
+  int device = -1;  // the context's tables, and its auxiliary streams, live on its device
+  if (hipGetDevice(&device) != hipSuccess || device != ctx->device) return BA_ERROR_INVALID_VALUE;
   std::vector<BaLeveledLaunch> leveled;
   rc = ba_build_lde(ctx, evals, in_stride, monomials, out, out_col_stride, out_coset_stride, log_n,
                     num_cols, log_blowup, leveled, 0);
   if (rc) return rc;
   std::vector<BaPassLaunch> flat;
   for (const BaLeveledLaunch &l : leveled) flat.push_back(l.launch);  // already in dependency order
-  return ba_run_launches(flat, (hipStream_t)stream);
+  // flat = the inverse NTT's passes, then each coset's: all the same count (one n2b plan per
+  // size class).  The coset NTTs only read the monomials, so cosets 1.. run on auxiliary streams
+  // forked after the inverse NTT and joined back: a pass's last partial wave of workgroups then
+  // overlaps another coset's passes (a graph captured from `stream` holds the same fork/join).
+  const uint32_t blowup = 1u << log_blowup;
+  const size_t per = flat.size() / (1 + blowup);
+  // (under launch tracing the launches stay in order on one stream, so each one's timing is its own)
+  if (blowup == 1 || per == 0 || per * (1 + blowup) != flat.size() || ba_trace_active())
+    return ba_run_launches(flat, (hipStream_t)stream);
+  const hipStream_t main = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lock(ctx->lde_mutex);
+  hipError_t err = hipSuccess;
+  if (!ctx->lde_fork) err = hipEventCreateWithFlags(&ctx->lde_fork, hipEventDisableTiming);
+  const int aux = (int)std::min<uint32_t>(blowup - 1, ba_context::kLdeAux);
+  for (int i = 0; err == hipSuccess && i < aux; i++) {
+    if (!ctx->lde_aux[i]) err = hipStreamCreateWithFlags(&ctx->lde_aux[i], hipStreamNonBlocking);
+    if (err == hipSuccess && !ctx->lde_join[i]) err = hipEventCreateWithFlags(&ctx->lde_join[i], hipEventDisableTiming);
+  }
+  if (err != hipSuccess) return (int)err;
+  rc = ba_run_launches(std::vector<BaPassLaunch>(flat.begin(), flat.begin() + per), main);
+  if (rc) return rc;
+  if ((err = hipEventRecord(ctx->lde_fork, main)) != hipSuccess) return (int)err;
+  for (int i = 0; i < aux; i++)
+    if ((err = hipStreamWaitEvent(ctx->lde_aux[i], ctx->lde_fork, 0)) != hipSuccess) return (int)err;
+  for (uint32_t c = 0; c < blowup; c++) {
+    const hipStream_t sc = c == 0 ? main : ctx->lde_aux[(c - 1) % aux];
+    const auto first = flat.begin() + per * (1 + c);
+    rc = ba_run_launches(std::vector<BaPassLaunch>(first, first + per), sc);
+    if (rc) return rc;
+  }
+  for (int i = 0; i < aux; i++) {
+    if ((err = hipEventRecord(ctx->lde_join[i], ctx->lde_aux[i])) != hipSuccess) return (int)err;
+    if ((err = hipStreamWaitEvent(main, ctx->lde_join[i], 0)) != hipSuccess) return (int)err;
+  }
+  return BA_SUCCESS;
 }

@@ -98,6 +98,8 @@ void ba_trace_before(hipStream_t stream, const char *kernel) {
   t.open = true;
 }
 
+bool ba_trace_active() { return g_trace.events != nullptr; }
+
 void ba_trace_after(hipStream_t stream) {
   TraceState &t = g_trace;
   if (!t.events || !t.open) return;
