@@ -37,7 +37,7 @@ struct ba_context {
   std::atomic<int> users{0};
   // Poseidon2 round constants (30 x 12, device memory), installed by the caller
   uint64_t *p2_round_constants = nullptr;
-  // ba_lde_batch runs the coset NTTs after the first on these streams (created on first use),
+  // ba_lde_batch runs the coset NTTs after the first on these streams (created with the context),
   // forked from and joined back into the caller's stream; the mutex keeps two callers' fork and
   // join events from interleaving
   static constexpr int kLdeAux = 7;

@@ -102,6 +102,23 @@ extern "C" int ba_context_create(uint32_t w_coarse_log, uint32_t g_coarse_log, b
   ctx->tab.g = {d + off_gf, d + off_gc, gf, (uint32_t)(n_gf - 1)};
   ctx->tab.g_inv = {d + off_if, d + off_ic, gf, (uint32_t)(n_gf - 1)};
   ctx->tab.inv_sizes = d + off_inv;
+  // the LDE's auxiliary streams and events, created here so that no LDE call creates any (it may
+  // run under stream capture); without them ba_lde_batch keeps every coset on the caller's stream
+  bool aux_ok = hipEventCreateWithFlags(&ctx->lde_fork, hipEventDisableTiming) == hipSuccess;
+  for (int i = 0; aux_ok && i < ba_context::kLdeAux; i++)
+    aux_ok = hipStreamCreateWithFlags(&ctx->lde_aux[i], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&ctx->lde_join[i], hipEventDisableTiming) == hipSuccess;
+  if (!aux_ok) {
+    for (int i = 0; i < ba_context::kLdeAux; i++) {
+      if (ctx->lde_aux[i]) (void)hipStreamDestroy(ctx->lde_aux[i]);
+      if (ctx->lde_join[i]) (void)hipEventDestroy(ctx->lde_join[i]);
+      ctx->lde_aux[i] = nullptr;
+      ctx->lde_join[i] = nullptr;
+    }
+    if (ctx->lde_fork) (void)hipEventDestroy(ctx->lde_fork);
+    ctx->lde_fork = nullptr;
+    (void)hipGetLastError();  // the context works without them
+  }
   {
     std::lock_guard<std::mutex> lock(g_registry_mutex);
     g_live.push_back(ctx);

@@ -111,18 +111,12 @@ extern "C" int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_strid
   const uint32_t blowup = 1u << log_blowup;
   const size_t per = flat.size() / (1 + blowup);
   // (under launch tracing the launches stay in order on one stream, so each one's timing is its own)
-  if (blowup == 1 || per == 0 || per * (1 + blowup) != flat.size() || ba_trace_active())
+  if (blowup == 1 || per == 0 || per * (1 + blowup) != flat.size() || ba_trace_active() || !ctx->lde_fork)
     return ba_run_launches(flat, (hipStream_t)stream);
   const hipStream_t main = (hipStream_t)stream;
   std::lock_guard<std::mutex> lock(ctx->lde_mutex);
   hipError_t err = hipSuccess;
-  if (!ctx->lde_fork) err = hipEventCreateWithFlags(&ctx->lde_fork, hipEventDisableTiming);
   const int aux = (int)std::min<uint32_t>(blowup - 1, ba_context::kLdeAux);
-  for (int i = 0; err == hipSuccess && i < aux; i++) {
-    if (!ctx->lde_aux[i]) err = hipStreamCreateWithFlags(&ctx->lde_aux[i], hipStreamNonBlocking);
-    if (err == hipSuccess && !ctx->lde_join[i]) err = hipEventCreateWithFlags(&ctx->lde_join[i], hipEventDisableTiming);
-  }
-  if (err != hipSuccess) return (int)err;
   rc = ba_run_launches(std::vector<BaPassLaunch>(flat.begin(), flat.begin() + per), main);
   if (rc) return rc;
   if ((err = hipEventRecord(ctx->lde_fork, main)) != hipSuccess) return (int)err;
