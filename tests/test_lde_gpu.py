@@ -200,13 +200,15 @@ def test_context_outlives_its_users(ba):
         ctx.destroy()
 
 
-def test_lde_graph_replay(ba):
+@pytest.mark.parametrize("lb", [2, 4])
+def test_lde_graph_replay(ba, lb):
     """The LDE captured as a hipGraph (as the benchmark runs it) gives the eager result on every
-    replay, with eager work and synchronisation between replays."""
+    replay, with eager work and synchronisation between replays.  Blowup 4 forks cosets 1..3 onto
+    three auxiliary streams; blowup 16 puts cosets 1..15 on all seven, two or three per stream."""
     import ctypes
 
-    log_n, cols, lb = 16, 6, 2
-    n, B = 1 << log_n, 4
+    log_n, cols = 16, 6
+    n, B = 1 << log_n, 1 << lb
     rng = np.random.default_rng(31)
     ev = raw_u64(rng, n * cols)
     d_ev, d_mono = ba.DeviceAllocation.from_host(ev), ba.DeviceAllocation(n * cols)
