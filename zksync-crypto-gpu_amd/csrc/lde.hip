@@ -120,17 +120,22 @@ extern "C" int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_strid
   rc = ba_run_launches(std::vector<BaPassLaunch>(flat.begin(), flat.begin() + per), main);
   if (rc) return rc;
   if ((err = hipEventRecord(ctx->lde_fork, main)) != hipSuccess) return (int)err;
-  for (int i = 0; i < aux; i++)
-    if ((err = hipStreamWaitEvent(ctx->lde_aux[i], ctx->lde_fork, 0)) != hipSuccess) return (int)err;
-  for (uint32_t c = 0; c < blowup; c++) {
+  // Once forked, every auxiliary stream that waited on the fork is joined back even when a later
+  // step fails, so that `stream` still orders after all enqueued work and a capture is not left
+  // with unjoined streams; the first error is returned.
+  int forked = 0;
+  for (; forked < aux && !rc; forked++)
+    if ((err = hipStreamWaitEvent(ctx->lde_aux[forked], ctx->lde_fork, 0)) != hipSuccess) rc = (int)err;
+  if (rc) forked--;  // the failed wait did not fork that stream
+  for (uint32_t c = 0; c < blowup && !rc; c++) {
     const hipStream_t sc = c == 0 ? main : ctx->lde_aux[(c - 1) % aux];
     const auto first = flat.begin() + per * (1 + c);
     rc = ba_run_launches(std::vector<BaPassLaunch>(first, first + per), sc);
-    if (rc) return rc;
   }
-  for (int i = 0; i < aux; i++) {
-    if ((err = hipEventRecord(ctx->lde_join[i], ctx->lde_aux[i])) != hipSuccess) return (int)err;
-    if ((err = hipStreamWaitEvent(main, ctx->lde_join[i], 0)) != hipSuccess) return (int)err;
+  for (int i = 0; i < forked; i++) {
+    if ((err = hipEventRecord(ctx->lde_join[i], ctx->lde_aux[i])) == hipSuccess)
+      err = hipStreamWaitEvent(main, ctx->lde_join[i], 0);
+    if (err != hipSuccess && !rc) rc = (int)err;
   }
-  return BA_SUCCESS;
+  return rc;
 }
