@@ -55,6 +55,10 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a 2^27-element copy kernel (canonicalise, 8 B read + 8 B written "
                          "per element) that scales the FETCH_SIZE/WRITE_SIZE counters")
+    ap.add_argument("--serial-cosets", action="store_true",
+                    help="run every LDE's coset NTTs in order on the caller's stream (sets "
+                         "BOOJUM_AMD_LDE_SERIAL=1), so that a profiler's per-kernel durations are each "
+                         "launch's own rather than overlapping the other cosets'")
     return ap.parse_args()
 
 
@@ -473,6 +477,8 @@ def main():
               "by columns with no communication; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.serial_cosets:
+        os.environ["BOOJUM_AMD_LDE_SERIAL"] = "1"  # read by the library at its first LDE
     import boojum_amd as ba
     from boojum_amd.profile import LaunchTrace
 
@@ -614,7 +620,8 @@ def main():
         "data": "synthetic: uniform field elements from a seeded SplitMix64 stream, generated in HBM",
         "config": {"workload": f"batched coset LDE, {cols} columns x 2^{log_n}, blowup {B}, shift 7, "
                                f"bit-reversed output, one hipGraph per step",
-                   "columns": cols, "log_n": log_n, "blowup": B, "parallelism": "single GPU"},
+                   "columns": cols, "log_n": log_n, "blowup": B, "parallelism": "single GPU",
+                   "cosets": "serial, one stream" if args.serial_cosets else "concurrent, auxiliary streams"},
         "lde_roofline": {"alg_bytes_per_step": lde_alg_bytes,
                          "achieved_GBs": round(lde_alg_bytes / (ms_per_step / 1e3) / 1e9, 1),
                          "frac": round(lde_alg_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},

@@ -150,8 +150,9 @@ int ba_bit_reverse_in_place(uint64_t *values, uint64_t stride, uint32_t log_rows
  * log_n + log_blowup <= 30, strides >= n, out distinct from evals / monomials, the context's
  * device current.  Cosets after the first run on auxiliary streams of the context, forked from
  * `stream` after the inverse NTT and joined back into it before the call returns (a graph
- * captured from `stream` holds the same fork and join; under ba_profile_begin every launch stays
- * on `stream`, in order); calls on one context are serialised
+ * captured from `stream` holds the same fork and join; under ba_profile_begin, or with
+ * BOOJUM_AMD_LDE_SERIAL=1 in the environment, every launch stays on `stream`, in order); calls
+ * on one context are serialised
  * while they enqueue, and no call on the context may run in another thread while a stream
  * capture holding one of its LDEs is still open. */
 int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_stride, uint64_t *monomials,

@@ -7,9 +7,11 @@
 //
 // Here the monomials stay in the bit-reversed order the inverse NTT leaves them in, and the
 // first pass of every coset NTT gathers them (coalesced chunk reads staged through LDS), so the
-// separate bit-reverse pass over the data disappears.  Launches: passes(iNTT) + B * passes(NTT),
-// all on one stream, capturable into a hipGraph.
+// separate bit-reverse pass over the data disappears.  Launches: passes(iNTT) + B * passes(NTT);
+// the inverse NTT and coset 0 on the caller's stream, cosets 1.. on the context's auxiliary
+// streams forked from it and joined back, capturable into a hipGraph.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "api_internal.h"
@@ -88,6 +90,16 @@ int ba_build_lde(const ba_context *ctx, uint64_t *evals, uint64_t in_stride, uin
   return BA_SUCCESS;
 }
 
+// BOOJUM_AMD_LDE_SERIAL=1 in the environment (read once) keeps every LDE's cosets in order on the
+// caller's stream, so that a profiler's per-kernel durations are each launch's own.
+static bool lde_serial() {
+  static const bool serial = [] {
+    const char *e = std::getenv("BOOJUM_AMD_LDE_SERIAL");
+    return e && e[0] == '1';
+  }();
+  return serial;
+}
+
 extern "C" int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_stride,
                             uint64_t *monomials, uint64_t *out, uint64_t out_col_stride,
                             uint64_t out_coset_stride, uint32_t log_n, uint32_t num_cols,
@@ -111,7 +123,8 @@ extern "C" int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_strid
   const uint32_t blowup = 1u << log_blowup;
   const size_t per = flat.size() / (1 + blowup);
   // (under launch tracing the launches stay in order on one stream, so each one's timing is its own)
-  if (blowup == 1 || per == 0 || per * (1 + blowup) != flat.size() || ba_trace_active() || !ctx->lde_fork)
+  if (blowup == 1 || per == 0 || per * (1 + blowup) != flat.size() || ba_trace_active() || !ctx->lde_fork ||
+      lde_serial())
     return ba_run_launches(flat, (hipStream_t)stream);
   const hipStream_t main = (hipStream_t)stream;
   std::lock_guard<std::mutex> lock(ctx->lde_mutex);
