@@ -114,3 +114,59 @@ def test_rust_sys_crate_declares_header():
         c_fields = re.findall(r"(\w+)(?:\[\d+\])?\s*[,;]", body)
         rust_body = re.search(r"pub struct %s \{(.*?)\}" % name, src, flags=re.S).group(1)
         assert re.findall(r"pub (\w+):", rust_body) == c_fields, name
+
+
+def _lde_rc(lib, evals, in_stride, mono, out, col_stride, coset_stride, log_n, cols, log_b):
+    return lib.ba_lde_batch(None, evals, in_stride, mono, out, col_stride, coset_stride, log_n, cols, log_b, None)
+
+
+def test_lde_output_layout_check_is_exact():
+    """ba_lde_batch checks its arguments on the host before it needs a context: with no context
+    a valid layout reports BA_ERROR_NOT_INITIALIZED (3) and an overlapping one
+    BA_ERROR_INVALID_VALUE (1).  The output vectors k*col_stride + c*coset_stride must be
+    pairwise disjoint; the library's answer equals a brute-force check over every pair of
+    vectors, over seeded small layouts (interleaved, coset-major, column-major, overlapping)."""
+    import random
+
+    from boojum_amd import _lib
+
+    lib = _lib.lib()
+    rng = random.Random(5)
+    evals, out = 1 << 40, 1 << 44  # addresses only: nothing is dereferenced
+    seen = {1: 0, 3: 0}
+    for _ in range(3000):
+        log_n = rng.randint(1, 4)
+        n = 1 << log_n
+        cols = rng.randint(1, 6)
+        log_b = rng.randint(1, 3)
+        b = 1 << log_b
+        col_stride = rng.choice([0, rng.randint(0, 4 * n * b), n * b, n, 2 * n, 3 * n])
+        coset_stride = rng.choice([0, rng.randint(0, 4 * n * b), n, n * cols, 2 * n, 3 * n])
+        starts = sorted(k * col_stride + c * coset_stride for k in range(cols) for c in range(b))
+        disjoint = all(y - x >= n for x, y in zip(starts, starts[1:]))
+        rc = _lde_rc(lib, evals, n, None, out, col_stride, coset_stride, log_n, cols, log_b)
+        assert rc == (3 if disjoint else 1), (log_n, cols, log_b, col_stride, coset_stride)
+        seen[rc] += 1
+    assert seen[1] > 100 and seen[3] > 100
+
+
+def test_lde_huge_layout_check_allocates_nothing():
+    """A legal call with 2^20 columns at blowup 2^20 (2^40 output vectors) is checked in time
+    linear in the smaller count, with no host list of starts: the old sorted list would have
+    needed 8 TB.  Outputs overlapping the evaluations are rejected."""
+    import time
+
+    from boojum_amd import _lib
+
+    lib = _lib.lib()
+    log_n, log_b, cols = 1, 20, 1 << 20
+    n = 1 << log_n
+    t0 = time.time()
+    rc = _lde_rc(lib, 1 << 40, n, None, 1 << 44, n << log_b, n, log_n, cols, log_b)
+    assert rc == 3 and time.time() - t0 < 5
+    # coset stride n - 1: neighbouring cosets overlap
+    assert _lde_rc(lib, 1 << 40, n, None, 1 << 44, n << log_b, n - 1, log_n, cols, log_b) == 1
+    # the evaluations inside the output range
+    assert _lde_rc(lib, (1 << 44) + 8, n, None, 1 << 44, n << log_b, n, log_n, cols, log_b) == 1
+    # an output range past the end of the address space
+    assert _lde_rc(lib, 1 << 40, n, None, (1 << 64) - (1 << 20), n << log_b, n, log_n, cols, log_b) == 1

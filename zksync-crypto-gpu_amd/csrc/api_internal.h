@@ -5,9 +5,24 @@
 
 #include <atomic>
 #include <mutex>
+#include <new>
 
 #include "../../include/boojum_amd.h"
 #include "gl_tables.h"
+
+// Runs the body of an extern "C" entry point so that no C++ exception crosses the C-ABI: host
+// allocation failures (planning vectors, server state, threads) become BA_ERROR_OUT_OF_MEMORY,
+// anything else BA_ERROR_INVALID_VALUE.
+template <typename F>
+int ba_guarded(F &&body) noexcept {
+  try {
+    return body();
+  } catch (const std::bad_alloc &) {
+    return BA_ERROR_OUT_OF_MEMORY;
+  } catch (...) {
+    return BA_ERROR_INVALID_VALUE;
+  }
+}
 
 // Launch-time plan of one NTT: the index bits each pass transforms, in execution order.
 struct NttPlan {

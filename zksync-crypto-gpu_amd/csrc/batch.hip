@@ -226,81 +226,89 @@ hipError_t ba_plan_launch_group(const BaGroup &g, const void *device, const BaPl
 
 extern "C" int ba_batch_plan_create(ba_context_t ctx, const ba_job *jobs, uint32_t num_jobs,
                                     ba_batch_plan_t *plan) {
-  if (!plan) return BA_ERROR_INVALID_VALUE;
-  *plan = nullptr;
-  if (!ctx) return BA_ERROR_NOT_INITIALIZED;
-  ba_batch_plan *p = new ba_batch_plan();
-  p->ctx = ctx;
-  ctx->users.fetch_add(1);
-  (void)hipGetDevice(&p->device);
-  int rc = ba_plan_image_build(ctx, jobs, num_jobs, p->image);
-  if (rc) {
-    delete p;
-    return rc;
-  }
-  hipError_t err = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
-  for (int i = 0; err == hipSuccess && i < ba_batch_plan::kAux; i++)
-    err = hipStreamCreateWithFlags(&p->aux[i], hipStreamNonBlocking);
-  p->done.resize(p->image.groups.size(), nullptr);
-  for (size_t i = 0; err == hipSuccess && i < p->done.size(); i++)
-    err = hipEventCreateWithFlags(&p->done[i], hipEventDisableTiming);
-  if (err == hipSuccess && p->image.bytes) {
-    err = hipMalloc(&p->mem, p->image.bytes);
-    if (err != hipSuccess) p->mem = nullptr;
-    if (err == hipSuccess) {
-      std::vector<char> host(p->image.bytes);
-      ba_plan_image_write(p->image, host.data(), p->mem);
-      err = hipMemcpy(p->mem, host.data(), host.size(), hipMemcpyHostToDevice);
+  return ba_guarded([&]() -> int {
+    if (!plan) return BA_ERROR_INVALID_VALUE;
+    *plan = nullptr;
+    if (!ctx) return BA_ERROR_NOT_INITIALIZED;
+    ba_batch_plan *p = new ba_batch_plan();
+    p->ctx = ctx;
+    ctx->users.fetch_add(1);
+    (void)hipGetDevice(&p->device);
+    int rc = ba_plan_image_build(ctx, jobs, num_jobs, p->image);
+    if (rc) {
+      delete p;
+      return rc;
     }
-  }
-  if (err != hipSuccess) {
-    delete p;
-    return (int)err;
-  }
-  *plan = p;
-  return BA_SUCCESS;
+    hipError_t err = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
+    for (int i = 0; err == hipSuccess && i < ba_batch_plan::kAux; i++)
+      err = hipStreamCreateWithFlags(&p->aux[i], hipStreamNonBlocking);
+    p->done.resize(p->image.groups.size(), nullptr);
+    for (size_t i = 0; err == hipSuccess && i < p->done.size(); i++)
+      err = hipEventCreateWithFlags(&p->done[i], hipEventDisableTiming);
+    if (err == hipSuccess && p->image.bytes) {
+      err = hipMalloc(&p->mem, p->image.bytes);
+      if (err != hipSuccess) p->mem = nullptr;
+      if (err == hipSuccess) {
+        std::vector<char> host(p->image.bytes);
+        ba_plan_image_write(p->image, host.data(), p->mem);
+        err = hipMemcpy(p->mem, host.data(), host.size(), hipMemcpyHostToDevice);
+      }
+    }
+    if (err != hipSuccess) {
+      delete p;
+      return (int)err;
+    }
+    *plan = p;
+    return BA_SUCCESS;
+  });
 }
 
 extern "C" int ba_batch_plan_launch(ba_batch_plan_t plan, ba_stream_t stream) {
-  if (!plan) return BA_ERROR_INVALID_VALUE;
-  hipStream_t s = (hipStream_t)stream;
-  const std::vector<BaGroup> &groups = plan->image.groups;
-  size_t i = 0;
-  while (i < groups.size()) {
-    size_t e = i;
-    while (e < groups.size() && groups[e].level == groups[i].level) e++;
-    hipError_t err = hipSuccess;
-    if (e - i == 1) {
-      err = ba_plan_launch_group(groups[i], plan->mem, plan->image, s);
-    } else {
-      // the largest group stays on the caller's stream, the others fork and join
-      err = hipEventRecord(plan->fork, s);
-      for (size_t k = i + 1; err == hipSuccess && k < e; k++) {
-        hipStream_t a = plan->aux[(k - i - 1) % ba_batch_plan::kAux];
-        err = hipStreamWaitEvent(a, plan->fork, 0);
-        if (err == hipSuccess) err = ba_plan_launch_group(groups[k], plan->mem, plan->image, a);
-        if (err == hipSuccess) err = hipEventRecord(plan->done[k], a);
+  return ba_guarded([&]() -> int {
+    if (!plan) return BA_ERROR_INVALID_VALUE;
+    hipStream_t s = (hipStream_t)stream;
+    const std::vector<BaGroup> &groups = plan->image.groups;
+    size_t i = 0;
+    while (i < groups.size()) {
+      size_t e = i;
+      while (e < groups.size() && groups[e].level == groups[i].level) e++;
+      hipError_t err = hipSuccess;
+      if (e - i == 1) {
+        err = ba_plan_launch_group(groups[i], plan->mem, plan->image, s);
+      } else {
+        // the largest group stays on the caller's stream, the others fork and join
+        err = hipEventRecord(plan->fork, s);
+        for (size_t k = i + 1; err == hipSuccess && k < e; k++) {
+          hipStream_t a = plan->aux[(k - i - 1) % ba_batch_plan::kAux];
+          err = hipStreamWaitEvent(a, plan->fork, 0);
+          if (err == hipSuccess) err = ba_plan_launch_group(groups[k], plan->mem, plan->image, a);
+          if (err == hipSuccess) err = hipEventRecord(plan->done[k], a);
+        }
+        if (err == hipSuccess) err = ba_plan_launch_group(groups[i], plan->mem, plan->image, s);
+        for (size_t k = i + 1; err == hipSuccess && k < e; k++) err = hipStreamWaitEvent(s, plan->done[k], 0);
       }
-      if (err == hipSuccess) err = ba_plan_launch_group(groups[i], plan->mem, plan->image, s);
-      for (size_t k = i + 1; err == hipSuccess && k < e; k++) err = hipStreamWaitEvent(s, plan->done[k], 0);
+      if (err != hipSuccess) return (int)err;
+      i = e;
     }
-    if (err != hipSuccess) return (int)err;
-    i = e;
-  }
-  return BA_SUCCESS;
+    return BA_SUCCESS;
+  });
 }
 
 extern "C" int ba_batch_plan_stats(ba_batch_plan_t plan, uint32_t *launches, uint32_t *levels,
                                    uint64_t *workgroups) {
-  if (!plan) return BA_ERROR_INVALID_VALUE;
-  if (launches) *launches = (uint32_t)plan->image.groups.size();
-  if (levels) *levels = plan->image.levels;
-  if (workgroups) *workgroups = plan->image.workgroups;
-  return BA_SUCCESS;
+  return ba_guarded([&]() -> int {
+    if (!plan) return BA_ERROR_INVALID_VALUE;
+    if (launches) *launches = (uint32_t)plan->image.groups.size();
+    if (levels) *levels = plan->image.levels;
+    if (workgroups) *workgroups = plan->image.workgroups;
+    return BA_SUCCESS;
+  });
 }
 
 extern "C" int ba_batch_plan_destroy(ba_batch_plan_t plan) {
-  if (!plan) return BA_ERROR_INVALID_VALUE;
-  delete plan;  // releases streams, events and the descriptor tables
-  return BA_SUCCESS;
+  return ba_guarded([&]() -> int {
+    if (!plan) return BA_ERROR_INVALID_VALUE;
+    delete plan;  // releases streams, events and the descriptor tables
+    return BA_SUCCESS;
+  });
 }

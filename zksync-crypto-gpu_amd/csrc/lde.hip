@@ -24,6 +24,35 @@ uint32_t bitrev_bits(uint32_t x, uint32_t bits) {
   return r;
 }
 
+// True when the n-element vectors starting at k s1 + c s2 (k < K, c < C) are pairwise disjoint.
+// Two starts differ by dk s1 + dc s2 with |dk| < K, |dc| < C, not both zero.  Up to sign the
+// index with the smaller count (outer) is >= 0; for each outer step the point of the inner
+// progression closest to it is found by division, so no list of starts is built and the work
+// is O(min(K, C)).
+bool vectors_disjoint(uint64_t K, uint64_t s1, uint64_t C, uint64_t s2, uint64_t n) {
+  typedef unsigned __int128 u128;
+  uint64_t O = K, so = s1, I = C, si = s2;
+  if (C < K) O = C, so = s2, I = K, si = s1;
+  if (I > 1 && si < n) return false;  // outer step 0: neighbours along the inner stride
+  for (uint64_t d = 1; d < O; d++) {
+    const u128 x = (u128)d * so;  // distance to cancel with inner steps -e, 0 <= e <= I - 1
+    u128 closest;
+    if (si == 0) {
+      closest = x;
+    } else {
+      const u128 q = x / si, r = x % si;
+      if (q > I - 1) {
+        closest = x - (u128)(I - 1) * si;
+      } else {
+        closest = r;
+        if (q + 1 <= I - 1 && (u128)si - r < closest) closest = (u128)si - r;
+      }
+    }
+    if (closest < n) return false;
+  }
+  return true;
+}
+
 }  // namespace
 
 int ba_validate_lde(uint64_t *evals, uint64_t in_stride, uint64_t *monomials, uint64_t *out,
@@ -32,29 +61,29 @@ int ba_validate_lde(uint64_t *evals, uint64_t in_stride, uint64_t *monomials, ui
   if (log_n < 1 || log_n > BA_MAX_LOG_N || log_blowup < 1 || log_n + log_blowup > gl::OMEGA_LOG_ORDER)
     return BA_ERROR_INVALID_VALUE;
   const uint64_t n = 1ull << log_n;
-  const uint32_t blowup = 1u << log_blowup;
+  const uint64_t blowup = 1ull << log_blowup;
   if (num_cols == 0) return BA_SUCCESS;
   if (!evals || !out || in_stride < n) return BA_ERROR_INVALID_VALUE;
+  // every buffer's extent must be addressable
+  typedef unsigned __int128 u128;
+  const u128 out_span = ((u128)(num_cols - 1) * out_col_stride + (u128)(blowup - 1) * out_coset_stride + n) * 8;
+  const u128 ev_span = ((u128)(num_cols - 1) * in_stride + n) * 8;
+  const u128 mono_span = (u128)num_cols * n * 8;
+  if ((u128)(uintptr_t)out + out_span > ((u128)1 << 64) || (u128)(uintptr_t)evals + ev_span > ((u128)1 << 64) ||
+      (monomials && (u128)(uintptr_t)monomials + mono_span > ((u128)1 << 64)))
+    return BA_ERROR_INVALID_VALUE;
   // the B * num_cols output vectors of n elements must be pairwise disjoint
-  {
-    std::vector<uint64_t> starts;
-    starts.reserve((size_t)num_cols * blowup);
-    for (uint64_t k = 0; k < num_cols; k++)
-      for (uint64_t c = 0; c < blowup; c++) starts.push_back(k * out_col_stride + c * out_coset_stride);
-    std::sort(starts.begin(), starts.end());
-    for (size_t i = 1; i < starts.size(); i++)
-      if (starts[i] - starts[i - 1] < n) return BA_ERROR_INVALID_VALUE;
-  }
+  if (!vectors_disjoint(num_cols, out_col_stride, blowup, out_coset_stride, n)) return BA_ERROR_INVALID_VALUE;
   // the outputs overlap neither the evaluations nor the monomials; monomials given separately
   // do not overlap the evaluations
-  const uint64_t *out_end = out + (num_cols - 1) * out_col_stride + (blowup - 1) * out_coset_stride + n;
-  const uint64_t *ev_end = evals + (num_cols - 1) * in_stride + n;
-  auto overlap = [](const uint64_t *a0, const uint64_t *a1, const uint64_t *b0, const uint64_t *b1) {
+  const char *out_end = (const char *)out + (size_t)out_span;
+  const char *ev_end = (const char *)evals + (size_t)ev_span;
+  auto overlap = [](const void *a0, const void *a1, const void *b0, const void *b1) {
     return a0 < b1 && b0 < a1;
   };
   if (overlap(out, out_end, evals, ev_end)) return BA_ERROR_INVALID_VALUE;
   if (monomials) {
-    const uint64_t *mono_end = monomials + (uint64_t)num_cols * n;
+    const char *mono_end = (const char *)monomials + (size_t)mono_span;
     if (overlap(out, out_end, monomials, mono_end) || overlap(evals, ev_end, monomials, mono_end))
       return BA_ERROR_INVALID_VALUE;
   }
@@ -104,51 +133,55 @@ extern "C" int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_strid
                             uint64_t *monomials, uint64_t *out, uint64_t out_col_stride,
                             uint64_t out_coset_stride, uint32_t log_n, uint32_t num_cols,
                             uint32_t log_blowup, ba_stream_t stream) {
-  if (!ctx) return BA_ERROR_NOT_INITIALIZED;
-  int rc = ba_validate_lde(evals, in_stride, monomials, out, out_col_stride, out_coset_stride, log_n,
-                           num_cols, log_blowup);
-  if (rc || num_cols == 0) return rc;
-  int device = -1;  // the context's tables, and its auxiliary streams, live on its device
-  if (hipGetDevice(&device) != hipSuccess || device != ctx->device) return BA_ERROR_INVALID_VALUE;
-  std::vector<BaLeveledLaunch> leveled;
-  rc = ba_build_lde(ctx, evals, in_stride, monomials, out, out_col_stride, out_coset_stride, log_n,
-                    num_cols, log_blowup, leveled, 0);
-  if (rc) return rc;
-  std::vector<BaPassLaunch> flat;
-  for (const BaLeveledLaunch &l : leveled) flat.push_back(l.launch);  // already in dependency order
-  // flat = the inverse NTT's passes, then each coset's: all the same count (one n2b plan per
-  // size class).  The coset NTTs only read the monomials, so cosets 1.. run on auxiliary streams
-  // forked after the inverse NTT and joined back: a pass's last partial wave of workgroups then
-  // overlaps another coset's passes (a graph captured from `stream` holds the same fork/join).
-  const uint32_t blowup = 1u << log_blowup;
-  const size_t per = flat.size() / (1 + blowup);
-  // (under launch tracing the launches stay in order on one stream, so each one's timing is its own)
-  if (blowup == 1 || per == 0 || per * (1 + blowup) != flat.size() || ba_trace_active() || !ctx->lde_fork ||
-      lde_serial())
-    return ba_run_launches(flat, (hipStream_t)stream);
-  const hipStream_t main = (hipStream_t)stream;
-  std::lock_guard<std::mutex> lock(ctx->lde_mutex);
-  hipError_t err = hipSuccess;
-  const int aux = (int)std::min<uint32_t>(blowup - 1, ba_context::kLdeAux);
-  rc = ba_run_launches(std::vector<BaPassLaunch>(flat.begin(), flat.begin() + per), main);
-  if (rc) return rc;
-  if ((err = hipEventRecord(ctx->lde_fork, main)) != hipSuccess) return (int)err;
-  // Once forked, every auxiliary stream that waited on the fork is joined back even when a later
-  // step fails, so that `stream` still orders after all enqueued work and a capture is not left
-  // with unjoined streams; the first error is returned.
-  int forked = 0;
-  for (; forked < aux && !rc; forked++)
-    if ((err = hipStreamWaitEvent(ctx->lde_aux[forked], ctx->lde_fork, 0)) != hipSuccess) rc = (int)err;
-  if (rc) forked--;  // the failed wait did not fork that stream
-  for (uint32_t c = 0; c < blowup && !rc; c++) {
-    const hipStream_t sc = c == 0 ? main : ctx->lde_aux[(c - 1) % aux];
-    const auto first = flat.begin() + per * (1 + c);
-    rc = ba_run_launches(std::vector<BaPassLaunch>(first, first + per), sc);
-  }
-  for (int i = 0; i < forked; i++) {
-    if ((err = hipEventRecord(ctx->lde_join[i], ctx->lde_aux[i])) == hipSuccess)
-      err = hipStreamWaitEvent(main, ctx->lde_join[i], 0);
-    if (err != hipSuccess && !rc) rc = (int)err;
-  }
-  return rc;
+  return ba_guarded([&]() -> int {
+    // arguments first (host-only checks), then the context
+    int rc = ba_validate_lde(evals, in_stride, monomials, out, out_col_stride, out_coset_stride, log_n,
+                             num_cols, log_blowup);
+    if (rc) return rc;
+    if (!ctx) return BA_ERROR_NOT_INITIALIZED;
+    if (num_cols == 0) return BA_SUCCESS;
+    int device = -1;  // the context's tables, and its auxiliary streams, live on its device
+    if (hipGetDevice(&device) != hipSuccess || device != ctx->device) return BA_ERROR_INVALID_VALUE;
+    std::vector<BaLeveledLaunch> leveled;
+    rc = ba_build_lde(ctx, evals, in_stride, monomials, out, out_col_stride, out_coset_stride, log_n,
+                      num_cols, log_blowup, leveled, 0);
+    if (rc) return rc;
+    std::vector<BaPassLaunch> flat;
+    for (const BaLeveledLaunch &l : leveled) flat.push_back(l.launch);  // already in dependency order
+    // flat = the inverse NTT's passes, then each coset's: all the same count (one n2b plan per
+    // size class).  The coset NTTs only read the monomials, so cosets 1.. run on auxiliary streams
+    // forked after the inverse NTT and joined back: a pass's last partial wave of workgroups then
+    // overlaps another coset's passes (a graph captured from `stream` holds the same fork/join).
+    const uint32_t blowup = 1u << log_blowup;
+    const size_t per = flat.size() / (1 + blowup);
+    // (under launch tracing the launches stay in order on one stream, so each one's timing is its own)
+    if (blowup == 1 || per == 0 || per * (1 + blowup) != flat.size() || ba_trace_active() || !ctx->lde_fork ||
+        lde_serial())
+      return ba_run_launches(flat, (hipStream_t)stream);
+    const hipStream_t main = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lock(ctx->lde_mutex);
+    hipError_t err = hipSuccess;
+    const int aux = (int)std::min<uint32_t>(blowup - 1, ba_context::kLdeAux);
+    rc = ba_run_launches(std::vector<BaPassLaunch>(flat.begin(), flat.begin() + per), main);
+    if (rc) return rc;
+    if ((err = hipEventRecord(ctx->lde_fork, main)) != hipSuccess) return (int)err;
+    // Once forked, every auxiliary stream that waited on the fork is joined back even when a later
+    // step fails, so that `stream` still orders after all enqueued work and a capture is not left
+    // with unjoined streams; the first error is returned.
+    int forked = 0;
+    for (; forked < aux && !rc; forked++)
+      if ((err = hipStreamWaitEvent(ctx->lde_aux[forked], ctx->lde_fork, 0)) != hipSuccess) rc = (int)err;
+    if (rc) forked--;  // the failed wait did not fork that stream
+    for (uint32_t c = 0; c < blowup && !rc; c++) {
+      const hipStream_t sc = c == 0 ? main : ctx->lde_aux[(c - 1) % aux];
+      const auto first = flat.begin() + per * (1 + c);
+      rc = ba_run_launches(std::vector<BaPassLaunch>(first, first + per), sc);
+    }
+    for (int i = 0; i < forked; i++) {
+      if ((err = hipEventRecord(ctx->lde_join[i], ctx->lde_aux[i])) == hipSuccess)
+        err = hipStreamWaitEvent(main, ctx->lde_join[i], 0);
+      if (err != hipSuccess && !rc) rc = (int)err;
+    }
+    return rc;
+  });
 }

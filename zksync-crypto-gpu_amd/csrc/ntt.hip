@@ -91,29 +91,32 @@ BaKernels pick_kernel(bool contig, uint32_t S, bool n2b, bool inverse, bool top,
   return ba_pick_strided(S, n2b, inverse, top, scale != 0, gather);
 }
 
-// kernel names as rocprofv3 prints them, for launch tracing
-const char *kernel_name(bool contig, uint32_t S, bool n2b, bool inverse, bool top, int scale,
-                        bool gather) {
-  static char names[2][13][2][2][2][3][2][120];
-  static bool ready = false;
-  static const char *tf[2] = {"false", "true"};
-  if (!ready) {
+// kernel names as rocprofv3 prints them, for launch tracing; the table is built once, by the
+// first caller (a function-local static initialiser, so concurrent first calls are safe)
+struct KernelNames {
+  char n[2][13][2][2][2][3][2][120];
+  KernelNames() {
+    static const char *tf[2] = {"false", "true"};
     for (int s = 0; s < 13; s++)
       for (int d = 0; d < 2; d++)
         for (int i = 0; i < 2; i++)
           for (int tp = 0; tp < 2; tp++)
             for (int sc = 0; sc < 3; sc++)
               for (int ga = 0; ga < 2; ga++) {
-                snprintf(names[1][s][d][i][tp][sc][ga], sizeof(names[1][s][d][i][tp][sc][ga]),
+                snprintf(n[1][s][d][i][tp][sc][ga], sizeof(n[1][s][d][i][tp][sc][ga]),
                          "void gl::ntt::ntt_pass_contig<%d, %s, %s, %d, %s>(gl::ntt::PassArgs)", s, tf[d],
                          tf[i], sc, tf[tp]);
-                snprintf(names[0][s][d][i][tp][sc][ga], sizeof(names[0][s][d][i][tp][sc][ga]),
+                snprintf(n[0][s][d][i][tp][sc][ga], sizeof(n[0][s][d][i][tp][sc][ga]),
                          "void gl::ntt::ntt_pass_strided<%d, %s, %s, %s, %s, %s>(gl::ntt::PassArgs)", s,
                          tf[d], tf[i], tf[tp], tf[sc != 0], tf[ga]);
               }
-    ready = true;
   }
-  return names[contig][S][n2b][inverse][top][scale][gather];
+};
+
+const char *kernel_name(bool contig, uint32_t S, bool n2b, bool inverse, bool top, int scale,
+                        bool gather) {
+  static const KernelNames names;
+  return names.n[contig][S][n2b][inverse][top][scale][gather];
 }
 
 constexpr uint64_t kMaxGrid = 0x7FFFFFFFull;
@@ -260,13 +263,15 @@ extern "C" int ba_batch_ntt_ctx(ba_context_t ctx, const uint64_t *inputs, uint64
                                 uint64_t stride_out, int bitrev_inputs, int inverse,
                                 uint32_t log_extension_degree, uint32_t coset_index,
                                 ba_stream_t stream) {
-  if (!ctx) return BA_ERROR_NOT_INITIALIZED;
-  int rc = ba_validate_ntt(log_n, stride_in, stride_out, log_extension_degree, coset_index);
-  if (rc) return rc;
-  if (num_ntts && (!inputs || !outputs)) return BA_ERROR_INVALID_VALUE;
-  return ba_launch_ntt(ctx, inputs, outputs, log_n, num_ntts, stride_in, stride_out,
-                       bitrev_inputs != 0, inverse != 0, log_extension_degree, coset_index,
-                       (hipStream_t)stream);
+  return ba_guarded([&]() -> int {
+    if (!ctx) return BA_ERROR_NOT_INITIALIZED;
+    int rc = ba_validate_ntt(log_n, stride_in, stride_out, log_extension_degree, coset_index);
+    if (rc) return rc;
+    if (num_ntts && (!inputs || !outputs)) return BA_ERROR_INVALID_VALUE;
+    return ba_launch_ntt(ctx, inputs, outputs, log_n, num_ntts, stride_in, stride_out,
+                         bitrev_inputs != 0, inverse != 0, log_extension_degree, coset_index,
+                         (hipStream_t)stream);
+  });
 }
 
 extern "C" int ba_batch_ntt_internal(const uint64_t *inputs, uint64_t *outputs, uint32_t log_n,
@@ -274,9 +279,11 @@ extern "C" int ba_batch_ntt_internal(const uint64_t *inputs, uint64_t *outputs, 
                                      uint32_t stride_between_output_arrays, int bitrev_inputs,
                                      int inverse, uint32_t log_extension_degree,
                                      uint32_t coset_index, ba_stream_t stream) {
-  return ba_batch_ntt_ctx(ba_context_for_current_device(), inputs, outputs, log_n, num_ntts,
-                          stride_between_input_arrays, stride_between_output_arrays,
-                          bitrev_inputs, inverse, log_extension_degree, coset_index, stream);
+  return ba_guarded([&]() -> int {
+    return ba_batch_ntt_ctx(ba_context_for_current_device(), inputs, outputs, log_n, num_ntts,
+                            stride_between_input_arrays, stride_between_output_arrays,
+                            bitrev_inputs, inverse, log_extension_degree, coset_index, stream);
+  });
 }
 
 extern "C" int ba_batch_ntt_out_of_place(const uint64_t *inputs_matrix, size_t inputs_len,
@@ -287,23 +294,25 @@ extern "C" int ba_batch_ntt_out_of_place(const uint64_t *inputs_matrix, size_t i
                                          uint32_t stride_between_output_arrays, int bitrev_inputs,
                                          int inverse, uint32_t log_extension_degree,
                                          uint32_t coset_index, ba_stream_t stream) {
-  // ntt.rs:309-320, with 64-bit arithmetic in place of the reference's u32 products
-  if (log_n < 1 || log_n > BA_MAX_LOG_N) return BA_ERROR_INVALID_VALUE;
-  const uint64_t n = 1ull << log_n;
-  if ((uint64_t)inputs_offset + n > stride_between_input_arrays) return BA_ERROR_INVALID_VALUE;
-  if ((uint64_t)outputs_offset + n > stride_between_output_arrays) return BA_ERROR_INVALID_VALUE;
-  if ((uint64_t)inputs_len < (uint64_t)num_ntts * stride_between_input_arrays) return BA_ERROR_INVALID_VALUE;
-  if ((uint64_t)outputs_len < (uint64_t)num_ntts * stride_between_output_arrays) return BA_ERROR_INVALID_VALUE;
-  // the reference's slices are distinct borrows (ntt.rs:317,320): no overlap at all
-  if (inputs_len && outputs_len && inputs_matrix < outputs_matrix + outputs_len &&
-      outputs_matrix < inputs_matrix + inputs_len)
-    return BA_ERROR_INVALID_VALUE;
-  const uint64_t *in = inputs_matrix + inputs_offset;
-  uint64_t *out = outputs_matrix + outputs_offset;
-  if ((const void *)in == (const void *)out) return BA_ERROR_INVALID_VALUE;
-  return ba_batch_ntt_internal(in, out, log_n, num_ntts, stride_between_input_arrays,
-                               stride_between_output_arrays, bitrev_inputs, inverse,
-                               log_extension_degree, coset_index, stream);
+  return ba_guarded([&]() -> int {
+    // ntt.rs:309-320, with 64-bit arithmetic in place of the reference's u32 products
+    if (log_n < 1 || log_n > BA_MAX_LOG_N) return BA_ERROR_INVALID_VALUE;
+    const uint64_t n = 1ull << log_n;
+    if ((uint64_t)inputs_offset + n > stride_between_input_arrays) return BA_ERROR_INVALID_VALUE;
+    if ((uint64_t)outputs_offset + n > stride_between_output_arrays) return BA_ERROR_INVALID_VALUE;
+    if ((uint64_t)inputs_len < (uint64_t)num_ntts * stride_between_input_arrays) return BA_ERROR_INVALID_VALUE;
+    if ((uint64_t)outputs_len < (uint64_t)num_ntts * stride_between_output_arrays) return BA_ERROR_INVALID_VALUE;
+    // the reference's slices are distinct borrows (ntt.rs:317,320): no overlap at all
+    if (inputs_len && outputs_len && inputs_matrix < outputs_matrix + outputs_len &&
+        outputs_matrix < inputs_matrix + inputs_len)
+      return BA_ERROR_INVALID_VALUE;
+    const uint64_t *in = inputs_matrix + inputs_offset;
+    uint64_t *out = outputs_matrix + outputs_offset;
+    if ((const void *)in == (const void *)out) return BA_ERROR_INVALID_VALUE;
+    return ba_batch_ntt_internal(in, out, log_n, num_ntts, stride_between_input_arrays,
+                                 stride_between_output_arrays, bitrev_inputs, inverse,
+                                 log_extension_degree, coset_index, stream);
+  });
 }
 
 extern "C" int ba_batch_ntt_in_place(uint64_t *inputs_matrix, size_t inputs_len, uint32_t log_n,
@@ -311,13 +320,15 @@ extern "C" int ba_batch_ntt_in_place(uint64_t *inputs_matrix, size_t inputs_len,
                                      uint32_t stride_between_input_arrays, int bitrev_inputs,
                                      int inverse, uint32_t log_extension_degree,
                                      uint32_t coset_index, ba_stream_t stream) {
-  // ntt.rs:350-352
-  if (log_n < 1 || log_n > BA_MAX_LOG_N) return BA_ERROR_INVALID_VALUE;
-  const uint64_t n = 1ull << log_n;
-  if ((uint64_t)inputs_offset + n > stride_between_input_arrays) return BA_ERROR_INVALID_VALUE;
-  if ((uint64_t)inputs_len < (uint64_t)num_ntts * stride_between_input_arrays) return BA_ERROR_INVALID_VALUE;
-  uint64_t *io = inputs_matrix + inputs_offset;
-  return ba_batch_ntt_internal(io, io, log_n, num_ntts, stride_between_input_arrays,
-                               stride_between_input_arrays, bitrev_inputs, inverse,
-                               log_extension_degree, coset_index, stream);
+  return ba_guarded([&]() -> int {
+    // ntt.rs:350-352
+    if (log_n < 1 || log_n > BA_MAX_LOG_N) return BA_ERROR_INVALID_VALUE;
+    const uint64_t n = 1ull << log_n;
+    if ((uint64_t)inputs_offset + n > stride_between_input_arrays) return BA_ERROR_INVALID_VALUE;
+    if ((uint64_t)inputs_len < (uint64_t)num_ntts * stride_between_input_arrays) return BA_ERROR_INVALID_VALUE;
+    uint64_t *io = inputs_matrix + inputs_offset;
+    return ba_batch_ntt_internal(io, io, log_n, num_ntts, stride_between_input_arrays,
+                                 stride_between_input_arrays, bitrev_inputs, inverse,
+                                 log_extension_degree, coset_index, stream);
+  });
 }

@@ -229,85 +229,97 @@ int run_node_layers(const ba_context *ctx, const u64 *values, size_t values_len,
 }  // namespace
 
 extern "C" int ba_poseidon2_gl_set_round_constants(ba_context_t ctx, const uint64_t *round_constants) {
-  if (!ctx) return BA_ERROR_NOT_INITIALIZED;
-  if (!round_constants) return BA_ERROR_INVALID_VALUE;
-  int device = -1;
-  if (hipGetDevice(&device) != hipSuccess || device != ctx->device) return BA_ERROR_INVALID_VALUE;
-  std::vector<u64> host(round_constants, round_constants + ROUNDS * WIDTH);
-  for (u64 &x : host) x = gl::canon(x);
-  if (!ctx->p2_round_constants) {
-    void *p = nullptr;
-    const hipError_t err = hipMalloc(&p, host.size() * sizeof(u64));
-    if (err != hipSuccess) return err == hipErrorOutOfMemory ? BA_ERROR_OUT_OF_MEMORY : (int)err;
-    ctx->p2_round_constants = (u64 *)p;
-  }
-  return (int)hipMemcpy(ctx->p2_round_constants, host.data(), host.size() * sizeof(u64), hipMemcpyHostToDevice);
+  return ba_guarded([&]() -> int {
+    if (!ctx) return BA_ERROR_NOT_INITIALIZED;
+    if (!round_constants) return BA_ERROR_INVALID_VALUE;
+    int device = -1;
+    if (hipGetDevice(&device) != hipSuccess || device != ctx->device) return BA_ERROR_INVALID_VALUE;
+    std::vector<u64> host(round_constants, round_constants + ROUNDS * WIDTH);
+    for (u64 &x : host) x = gl::canon(x);
+    if (!ctx->p2_round_constants) {
+      void *p = nullptr;
+      const hipError_t err = hipMalloc(&p, host.size() * sizeof(u64));
+      if (err != hipSuccess) return err == hipErrorOutOfMemory ? BA_ERROR_OUT_OF_MEMORY : (int)err;
+      ctx->p2_round_constants = (u64 *)p;
+    }
+    return (int)hipMemcpy(ctx->p2_round_constants, host.data(), host.size() * sizeof(u64), hipMemcpyHostToDevice);
+  });
 }
 
 extern "C" int ba_poseidon2_gl_compute_leaf_hashes(const uint64_t *values, size_t values_len, uint64_t *results,
                                                    size_t results_len, uint32_t log_rows_per_hash,
                                                    int load_intermediate, int store_intermediate, ba_stream_t stream) {
-  ba_context *ctx = nullptr;
-  if (const int rc = hasher_context(ctx)) return rc;
-  // poseidon2.rs:186-207
-  if (results_len % CAPACITY != 0 || log_rows_per_hash >= 32) return BA_ERROR_INVALID_VALUE;
-  const size_t count = results_len / CAPACITY;
-  if (count == 0) return BA_ERROR_INVALID_VALUE;
-  const size_t rows = (size_t)1 << log_rows_per_hash;
-  if (values_len % (count * rows) != 0) return BA_ERROR_INVALID_VALUE;
-  const size_t cols = values_len / (count * rows);
-  if (store_intermediate && (rows * cols) % RATE != 0) return BA_ERROR_INVALID_VALUE;
-  if (cols > 0xFFFFFFFFull || count > 0xFFFFFFFFull || (values_len && !values) || !results)
-    return BA_ERROR_INVALID_VALUE;
-  leaves_kernel<<<(u32)((count + THREADS - 1) / THREADS), THREADS, 0, (hipStream_t)stream>>>(
-      values, results, (u32)rows, (u32)cols, (u32)count, load_intermediate, store_intermediate,
-      ctx->p2_round_constants);
-  return (int)hipGetLastError();
+  return ba_guarded([&]() -> int {
+    ba_context *ctx = nullptr;
+    if (const int rc = hasher_context(ctx)) return rc;
+    // poseidon2.rs:186-207
+    if (results_len % CAPACITY != 0 || log_rows_per_hash >= 32) return BA_ERROR_INVALID_VALUE;
+    const size_t count = results_len / CAPACITY;
+    if (count == 0) return BA_ERROR_INVALID_VALUE;
+    const size_t rows = (size_t)1 << log_rows_per_hash;
+    if (values_len % (count * rows) != 0) return BA_ERROR_INVALID_VALUE;
+    const size_t cols = values_len / (count * rows);
+    if (store_intermediate && (rows * cols) % RATE != 0) return BA_ERROR_INVALID_VALUE;
+    if (cols > 0xFFFFFFFFull || count > 0xFFFFFFFFull || (values_len && !values) || !results)
+      return BA_ERROR_INVALID_VALUE;
+    leaves_kernel<<<(u32)((count + THREADS - 1) / THREADS), THREADS, 0, (hipStream_t)stream>>>(
+        values, results, (u32)rows, (u32)cols, (u32)count, load_intermediate, store_intermediate,
+        ctx->p2_round_constants);
+    return (int)hipGetLastError();
+  });
 }
 
 extern "C" int ba_poseidon2_gl_compute_node_hashes(const uint64_t *values, size_t values_len, uint64_t *results,
                                                    size_t results_len, ba_stream_t stream) {
-  ba_context *ctx = nullptr;
-  if (const int rc = hasher_context(ctx)) return rc;
-  if (!nodes_shape_ok(values_len, results_len) || (results_len && (!values || !results)))
-    return BA_ERROR_INVALID_VALUE;
-  return launch_nodes(ctx, values, results, (u32)(results_len / CAPACITY), (hipStream_t)stream);
+  return ba_guarded([&]() -> int {
+    ba_context *ctx = nullptr;
+    if (const int rc = hasher_context(ctx)) return rc;
+    if (!nodes_shape_ok(values_len, results_len) || (results_len && (!values || !results)))
+      return BA_ERROR_INVALID_VALUE;
+    return launch_nodes(ctx, values, results, (u32)(results_len / CAPACITY), (hipStream_t)stream);
+  });
 }
 
 extern "C" int ba_poseidon2_gl_build_merkle_tree_leaves(const uint64_t *values, size_t values_len, uint64_t *results,
                                                         size_t results_len, uint32_t log_rows_per_hash,
                                                         int load_intermediate, int store_intermediate,
                                                         ba_stream_t stream) {
-  // poseidon2.rs:253-274: the leaves count divides the values
-  if (results_len % CAPACITY != 0 || results_len == 0 || values_len % (results_len / CAPACITY) != 0)
-    return BA_ERROR_INVALID_VALUE;
-  return ba_poseidon2_gl_compute_leaf_hashes(values, values_len, results, results_len, log_rows_per_hash,
-                                             load_intermediate, store_intermediate, stream);
+  return ba_guarded([&]() -> int {
+    // poseidon2.rs:253-274: the leaves count divides the values
+    if (results_len % CAPACITY != 0 || results_len == 0 || values_len % (results_len / CAPACITY) != 0)
+      return BA_ERROR_INVALID_VALUE;
+    return ba_poseidon2_gl_compute_leaf_hashes(values, values_len, results, results_len, log_rows_per_hash,
+                                               load_intermediate, store_intermediate, stream);
+  });
 }
 
 extern "C" int ba_poseidon2_gl_build_merkle_tree_nodes(const uint64_t *values, size_t values_len, uint64_t *results,
                                                        size_t results_len, uint32_t layers_count, ba_stream_t stream) {
-  ba_context *ctx = nullptr;
-  if (const int rc = hasher_context(ctx)) return rc;
-  if (layers_count == 0) return BA_SUCCESS;
-  if (!node_layers_ok(values_len, results_len, layers_count) || !values || !results) return BA_ERROR_INVALID_VALUE;
-  return run_node_layers(ctx, values, values_len, results, results_len, layers_count, (hipStream_t)stream);
+  return ba_guarded([&]() -> int {
+    ba_context *ctx = nullptr;
+    if (const int rc = hasher_context(ctx)) return rc;
+    if (layers_count == 0) return BA_SUCCESS;
+    if (!node_layers_ok(values_len, results_len, layers_count) || !values || !results) return BA_ERROR_INVALID_VALUE;
+    return run_node_layers(ctx, values, values_len, results, results_len, layers_count, (hipStream_t)stream);
+  });
 }
 
 extern "C" int ba_poseidon2_gl_build_merkle_tree(const uint64_t *values, size_t values_len, uint64_t *results,
                                                  size_t results_len, uint32_t log_rows_per_hash, uint32_t layers_count,
                                                  ba_stream_t stream) {
-  ba_context *ctx = nullptr;
-  if (const int rc = hasher_context(ctx)) return rc;
-  // poseidon2.rs:297-316
-  if (layers_count == 0 || layers_count > 40 || results_len % (2 * CAPACITY) != 0) return BA_ERROR_INVALID_VALUE;
-  const size_t leaves_count = results_len / (2 * CAPACITY);
-  if (leaves_count == 0 || ((size_t)1 << (layers_count - 1)) > leaves_count || values_len % leaves_count != 0)
-    return BA_ERROR_INVALID_VALUE;
-  const size_t half = results_len >> 1;
-  // every node layer is checked before the leaves launch, so a bad shape launches nothing
-  if (!node_layers_ok(half, results_len - half, layers_count - 1)) return BA_ERROR_INVALID_VALUE;
-  const int rc = ba_poseidon2_gl_compute_leaf_hashes(values, values_len, results, half, log_rows_per_hash, 0, 0, stream);
-  if (rc) return rc;
-  return run_node_layers(ctx, results, half, results + half, results_len - half, layers_count - 1, (hipStream_t)stream);
+  return ba_guarded([&]() -> int {
+    ba_context *ctx = nullptr;
+    if (const int rc = hasher_context(ctx)) return rc;
+    // poseidon2.rs:297-316
+    if (layers_count == 0 || layers_count > 40 || results_len % (2 * CAPACITY) != 0) return BA_ERROR_INVALID_VALUE;
+    const size_t leaves_count = results_len / (2 * CAPACITY);
+    if (leaves_count == 0 || ((size_t)1 << (layers_count - 1)) > leaves_count || values_len % leaves_count != 0)
+      return BA_ERROR_INVALID_VALUE;
+    const size_t half = results_len >> 1;
+    // every node layer is checked before the leaves launch, so a bad shape launches nothing
+    if (!node_layers_ok(half, results_len - half, layers_count - 1)) return BA_ERROR_INVALID_VALUE;
+    const int rc = ba_poseidon2_gl_compute_leaf_hashes(values, values_len, results, half, log_rows_per_hash, 0, 0, stream);
+    if (rc) return rc;
+    return run_node_layers(ctx, results, half, results + half, results_len - half, layers_count - 1, (hipStream_t)stream);
+  });
 }

@@ -109,18 +109,22 @@ void ba_trace_after(hipStream_t stream) {
 }
 
 extern "C" int ba_profile_begin(ba_event_t *events, uint32_t capacity) {
-  if (!events || capacity == 0) return BA_ERROR_INVALID_VALUE;
-  g_trace = TraceState{};
-  g_trace.events = (hipEvent_t *)events;
-  g_trace.capacity = capacity;
-  return BA_SUCCESS;
+  return ba_guarded([&]() -> int {
+    if (!events || capacity == 0) return BA_ERROR_INVALID_VALUE;
+    g_trace = TraceState{};
+    g_trace.events = (hipEvent_t *)events;
+    g_trace.capacity = capacity;
+    return BA_SUCCESS;
+  });
 }
 
 extern "C" int ba_profile_end(uint32_t *launches) {
-  if (launches) *launches = g_trace.count;
-  g_trace.events = nullptr;
-  g_trace.capacity = 0;
-  return BA_SUCCESS;
+  return ba_guarded([&]() -> int {
+    if (launches) *launches = g_trace.count;
+    g_trace.events = nullptr;
+    g_trace.capacity = 0;
+    return BA_SUCCESS;
+  });
 }
 
 extern "C" const char *ba_profile_launch_name(uint32_t index) {

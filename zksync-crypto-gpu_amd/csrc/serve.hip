@@ -86,7 +86,7 @@ struct Request {
   State state = State::Queued;
   int status = 0;
   size_t work_off = 0, out_off = 0, work_bytes = 0, out_bytes = 0;
-  Clock::time_point submit_t, admit_t, done_t;
+  Clock::time_point submit_t, admit_t, launch_t, done_t;
   uint64_t batch = 0;
 };
 
@@ -129,6 +129,7 @@ struct ba_server {
 
   void loop();
   void finish(Slot &s, int status);
+  void fail(Request &r, int status, Clock::time_point now);
   bool launch(Slot &s, std::unique_lock<std::mutex> &lk);
 };
 
@@ -147,11 +148,7 @@ void ba_server::finish(Slot &s, int status) {
       out_total += out_elems(r.req);
       latencies.push_back(ms_between(r.submit_t, now));
     } else {
-      r.state = State::Failed;
-      r.status = status;
-      failed++;
-      arena.release(r.out_off, r.out_bytes);
-      r.out_bytes = 0;
+      fail(r, status, now);
     }
   }
   last_done = now;
@@ -160,8 +157,44 @@ void ba_server::finish(Slot &s, int status) {
   cv_done.notify_all();
 }
 
+// The batch-plan job of one admitted request, with its buffers in the arena at `base`.
+static ba_job request_job(const Request &r, void *base) {
+  const ba_serve_request &q = r.req;
+  const uint64_t n = 1ull << q.log_n;
+  uint64_t *work = (uint64_t *)((char *)base + r.work_off);
+  ba_job j{};
+  j.kind = BA_JOB_LDE;
+  j.log_n = q.log_n;
+  j.num_cols = q.num_cols;
+  j.log_blowup = q.log_blowup;
+  j.outputs = (uint64_t *)((char *)base + r.out_off);
+  j.stride_out = n << q.log_blowup;
+  j.out_coset_stride = n;
+  if (q.input_kind == BA_SERVE_INPUT_DEVICE) {  // the caller's evaluations stay untouched
+    j.inputs = const_cast<uint64_t *>(q.evals);
+    j.stride_in = q.in_stride;
+    j.monomials = work;
+  } else {
+    j.inputs = work;
+    j.stride_in = n;
+  }
+  return j;
+}
+
+void ba_server::fail(Request &r, int status, Clock::time_point now) {
+  if (r.work_bytes) arena.release(r.work_off, r.work_bytes);
+  if (r.out_bytes) arena.release(r.out_off, r.out_bytes);
+  r.work_bytes = r.out_bytes = 0;
+  r.state = State::Failed;
+  r.status = status;
+  r.done_t = now;
+  failed++;
+}
+
 // Enqueues every admitted request (as many as the plan image capacity allows) on slot `s`.
-// Called with the lock held; releases it while planning and enqueueing.
+// Called with the lock held; releases it while planning and enqueueing.  Each request is first
+// validated on its own, so that one client's invalid request (say, device evaluations that
+// overlap the arena) fails only itself and not the requests batched with it.
 bool ba_server::launch(Slot &s, std::unique_lock<std::mutex> &lk) {
   std::vector<uint64_t> take(admitted.begin(), admitted.end());
   std::vector<Request> rs;
@@ -169,46 +202,35 @@ bool ba_server::launch(Slot &s, std::unique_lock<std::mutex> &lk) {
   const uint64_t batch = batches++;
   s.busy = true;
   lk.unlock();
+  std::vector<size_t> good;
+  std::vector<std::pair<uint64_t, int>> bad;
+  for (size_t i = 0; i < rs.size(); i++) {
+    const ba_job j = request_job(rs[i], base);
+    const int v = ba_validate_lde(j.inputs, j.stride_in, j.monomials, j.outputs, j.stride_out, j.out_coset_stride,
+                                  j.log_n, j.num_cols, j.log_blowup);
+    if (v)
+      bad.push_back({take[i], v});
+    else
+      good.push_back(i);
+  }
   BaPlanImage img;
   std::vector<ba_job> jobs;
-  size_t count = rs.size();
+  size_t count = good.size();
   int rc = BA_SUCCESS;
-  for (;;) {  // the largest prefix of the admitted requests whose plan fits the slot
+  while (count) {  // the largest prefix of the valid requests whose plan fits the slot
     jobs.clear();
-    for (size_t i = 0; i < count; i++) {
-      const ba_serve_request &q = rs[i].req;
-      const uint64_t n = 1ull << q.log_n;
-      uint64_t *work = (uint64_t *)((char *)base + rs[i].work_off);
-      uint64_t *out = (uint64_t *)((char *)base + rs[i].out_off);
-      ba_job j{};
-      j.kind = BA_JOB_LDE;
-      j.log_n = q.log_n;
-      j.num_cols = q.num_cols;
-      j.log_blowup = q.log_blowup;
-      j.outputs = out;
-      j.stride_out = n << q.log_blowup;
-      j.out_coset_stride = n;
-      if (q.input_kind == BA_SERVE_INPUT_DEVICE) {  // the caller's evaluations stay untouched
-        j.inputs = const_cast<uint64_t *>(q.evals);
-        j.stride_in = q.in_stride;
-        j.monomials = work;
-      } else {
-        j.inputs = work;
-        j.stride_in = n;
-      }
-      jobs.push_back(j);
-    }
+    for (size_t k = 0; k < count; k++) jobs.push_back(request_job(rs[good[k]], base));
     rc = ba_plan_image_build(ctx, jobs.data(), (uint32_t)jobs.size(), img);
     if (rc || img.bytes <= kDescBytes || count == 1) break;
     count = (count + 1) / 2;
   }
   if (rc == BA_SUCCESS && img.bytes > kDescBytes) rc = BA_ERROR_INVALID_VALUE;
   hipError_t err = hipSuccess;
-  if (rc == BA_SUCCESS) {
-    for (size_t i = 0; err == hipSuccess && i < count; i++) {
-      const ba_serve_request &q = rs[i].req;
+  if (rc == BA_SUCCESS && count) {
+    for (size_t k = 0; err == hipSuccess && k < count; k++) {
+      const ba_serve_request &q = rs[good[k]].req;
       const uint64_t n = 1ull << q.log_n;
-      uint64_t *work = (uint64_t *)((char *)base + rs[i].work_off);
+      uint64_t *work = (uint64_t *)((char *)base + rs[good[k]].work_off);
       if (q.input_kind == BA_SERVE_INPUT_HOST)
         err = hipMemcpy2DAsync(work, n * 8, q.evals, q.in_stride * 8, n * 8, q.num_cols, hipMemcpyHostToDevice,
                                s.stream);
@@ -225,15 +247,31 @@ bool ba_server::launch(Slot &s, std::unique_lock<std::mutex> &lk) {
     rc = (int)err;
   }
   lk.lock();
+  const auto now = Clock::now();
+  std::vector<uint64_t> gone;  // leave the admitted list: the invalid ones and the launched prefix
+  for (const auto &b : bad) {
+    fail(reqs.at(b.first), b.second, now);
+    gone.push_back(b.first);
+  }
   s.batch = batch;
-  s.tickets.assign(take.begin(), take.begin() + count);
-  admitted.erase(admitted.begin(), admitted.begin() + count);
+  s.tickets.clear();
+  for (size_t k = 0; k < count; k++) s.tickets.push_back(take[good[k]]);
+  gone.insert(gone.end(), s.tickets.begin(), s.tickets.end());
+  admitted.erase(std::remove_if(admitted.begin(), admitted.end(),
+                                [&](uint64_t t) { return std::find(gone.begin(), gone.end(), t) != gone.end(); }),
+                 admitted.end());
   for (uint64_t t : s.tickets) {
     Request &r = reqs.at(t);
     r.state = State::Running;
     r.batch = batch;
+    r.launch_t = now;
   }
   batched_jobs += count;
+  if (!bad.empty()) cv_done.notify_all();
+  if (count == 0) {
+    s.busy = false;
+    return false;
+  }
   if (rc != BA_SUCCESS) {
     // nothing of this batch may still run when its buffers go back to the arena
     (void)hipStreamSynchronize(s.stream);
@@ -307,61 +345,65 @@ void ba_server::loop() {
 
 extern "C" int ba_server_create(ba_context_t ctx, size_t memory_budget_bytes, uint32_t max_inflight_batches,
                                 ba_server_t *out) {
-  if (!out) return BA_ERROR_INVALID_VALUE;
-  *out = nullptr;
-  if (!ctx) return BA_ERROR_NOT_INITIALIZED;
-  if (memory_budget_bytes < 4096 || max_inflight_batches < 1 || max_inflight_batches > 8)
-    return BA_ERROR_INVALID_VALUE;
-  int device = -1;
-  if (hipGetDevice(&device) != hipSuccess || device != ctx->device) return BA_ERROR_INVALID_VALUE;
-  ba_server *s = new ba_server(Arena::round(memory_budget_bytes));
-  s->ctx = ctx;
-  ctx->users.fetch_add(1);
-  s->device = device;
-  hipError_t err = hipMalloc(&s->base, s->arena.size());
-  if (err != hipSuccess) s->base = nullptr;
-  s->slots.resize(max_inflight_batches);
-  for (Slot &sl : s->slots) {
-    if (err == hipSuccess) err = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
-    if (err == hipSuccess) err = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
-    if (err == hipSuccess) err = hipHostMalloc(&sl.staging, kDescBytes, hipHostMallocDefault);
-    if (err == hipSuccess) err = hipMalloc(&sl.desc, kDescBytes);
-  }
-  if (err != hipSuccess) {
-    ba_server_destroy(s);
-    return err == hipErrorOutOfMemory ? BA_ERROR_OUT_OF_MEMORY : (int)err;
-  }
-  s->worker = std::thread([s] { s->loop(); });
-  *out = s;
-  return BA_SUCCESS;
+  return ba_guarded([&]() -> int {
+    if (!out) return BA_ERROR_INVALID_VALUE;
+    *out = nullptr;
+    if (!ctx) return BA_ERROR_NOT_INITIALIZED;
+    if (memory_budget_bytes < 4096 || max_inflight_batches < 1 || max_inflight_batches > 8)
+      return BA_ERROR_INVALID_VALUE;
+    int device = -1;
+    if (hipGetDevice(&device) != hipSuccess || device != ctx->device) return BA_ERROR_INVALID_VALUE;
+    ba_server *s = new ba_server(Arena::round(memory_budget_bytes));
+    s->ctx = ctx;
+    ctx->users.fetch_add(1);
+    s->device = device;
+    hipError_t err = hipMalloc(&s->base, s->arena.size());
+    if (err != hipSuccess) s->base = nullptr;
+    s->slots.resize(max_inflight_batches);
+    for (Slot &sl : s->slots) {
+      if (err == hipSuccess) err = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+      if (err == hipSuccess) err = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+      if (err == hipSuccess) err = hipHostMalloc(&sl.staging, kDescBytes, hipHostMallocDefault);
+      if (err == hipSuccess) err = hipMalloc(&sl.desc, kDescBytes);
+    }
+    if (err != hipSuccess) {
+      ba_server_destroy(s);
+      return err == hipErrorOutOfMemory ? BA_ERROR_OUT_OF_MEMORY : (int)err;
+    }
+    s->worker = std::thread([s] { s->loop(); });
+    *out = s;
+    return BA_SUCCESS;
+  });
 }
 
 extern "C" int ba_server_submit(ba_server_t s, const ba_serve_request *req, uint64_t *ticket) {
-  if (!s || !req || !ticket) return BA_ERROR_INVALID_VALUE;
-  const ba_serve_request &q = *req;
-  const uint64_t n = q.log_n <= BA_MAX_LOG_N ? 1ull << q.log_n : 0;
-  if (q.log_n < 1 || q.log_n > BA_MAX_LOG_N || q.num_cols == 0 || q.log_blowup < 1 ||
-      q.log_n + q.log_blowup > BA_OMEGA_LOG_ORDER || q.input_kind > BA_SERVE_INPUT_SYNTHETIC)
-    return BA_ERROR_INVALID_VALUE;
-  ba_serve_request r = q;
-  if (r.in_stride == 0) r.in_stride = n;
-  if (r.input_kind != BA_SERVE_INPUT_SYNTHETIC && (!r.evals || (r.num_cols > 1 && r.in_stride < n)))
-    return BA_ERROR_INVALID_VALUE;
-  Request rec;
-  rec.req = r;
-  rec.work_bytes = work_elems(r) * 8;
-  rec.out_bytes = out_elems(r) * 8;
-  if (Arena::round(rec.work_bytes) + Arena::round(rec.out_bytes) > s->arena.size()) return BA_ERROR_OUT_OF_MEMORY;
-  std::lock_guard<std::mutex> g(s->m);
-  if (s->stop) return BA_ERROR_INVALID_VALUE;
-  rec.submit_t = Clock::now();
-  if (!s->have_first) s->have_first = true, s->first_submit = rec.submit_t;
-  const uint64_t t = s->next_ticket++;
-  s->reqs.emplace(t, rec);
-  s->queued.push_back(t);
-  *ticket = t;
-  s->cv_work.notify_one();
-  return BA_SUCCESS;
+  return ba_guarded([&]() -> int {
+    if (!s || !req || !ticket) return BA_ERROR_INVALID_VALUE;
+    const ba_serve_request &q = *req;
+    const uint64_t n = q.log_n <= BA_MAX_LOG_N ? 1ull << q.log_n : 0;
+    if (q.log_n < 1 || q.log_n > BA_MAX_LOG_N || q.num_cols == 0 || q.log_blowup < 1 ||
+        q.log_n + q.log_blowup > BA_OMEGA_LOG_ORDER || q.input_kind > BA_SERVE_INPUT_SYNTHETIC)
+      return BA_ERROR_INVALID_VALUE;
+    ba_serve_request r = q;
+    // one column has no stride to speak of; several need columns at least n apart
+    if (r.in_stride == 0 || r.num_cols == 1) r.in_stride = n;
+    if (r.input_kind != BA_SERVE_INPUT_SYNTHETIC && (!r.evals || r.in_stride < n)) return BA_ERROR_INVALID_VALUE;
+    Request rec;
+    rec.req = r;
+    rec.work_bytes = work_elems(r) * 8;
+    rec.out_bytes = out_elems(r) * 8;
+    if (Arena::round(rec.work_bytes) + Arena::round(rec.out_bytes) > s->arena.size()) return BA_ERROR_OUT_OF_MEMORY;
+    std::lock_guard<std::mutex> g(s->m);
+    if (s->stop) return BA_ERROR_INVALID_VALUE;
+    rec.submit_t = Clock::now();
+    if (!s->have_first) s->have_first = true, s->first_submit = rec.submit_t;
+    const uint64_t t = s->next_ticket++;
+    s->reqs.emplace(t, rec);
+    s->queued.push_back(t);
+    *ticket = t;
+    s->cv_work.notify_one();
+    return BA_SUCCESS;
+  });
 }
 
 static void fill_result(ba_server *s, const Request &r, ba_serve_result *res) {
@@ -376,90 +418,100 @@ static void fill_result(ba_server *s, const Request &r, ba_serve_result *res) {
 }
 
 extern "C" int ba_server_wait(ba_server_t s, uint64_t ticket, ba_serve_result *res) {
-  if (!s) return BA_ERROR_INVALID_VALUE;
-  std::unique_lock<std::mutex> lk(s->m);
-  auto done = [&] {
+  return ba_guarded([&]() -> int {
+    if (!s) return BA_ERROR_INVALID_VALUE;
+    std::unique_lock<std::mutex> lk(s->m);
+    auto done = [&] {
+      auto it = s->reqs.find(ticket);
+      return it == s->reqs.end() || it->second.state == State::Done || it->second.state == State::Failed;
+    };
+    s->cv_done.wait(lk, done);
     auto it = s->reqs.find(ticket);
-    return it == s->reqs.end() || it->second.state == State::Done || it->second.state == State::Failed;
-  };
-  s->cv_done.wait(lk, done);
-  auto it = s->reqs.find(ticket);
-  if (it == s->reqs.end()) return BA_ERROR_INVALID_VALUE;  // unknown or already released
-  fill_result(s, it->second, res);
-  return BA_SUCCESS;
+    if (it == s->reqs.end()) return BA_ERROR_INVALID_VALUE;  // unknown or already released
+    fill_result(s, it->second, res);
+    return BA_SUCCESS;
+  });
 }
 
 extern "C" int ba_server_poll(ba_server_t s, uint64_t ticket, int *done, ba_serve_result *res) {
-  if (!s || !done) return BA_ERROR_INVALID_VALUE;
-  std::lock_guard<std::mutex> g(s->m);
-  auto it = s->reqs.find(ticket);
-  if (it == s->reqs.end()) return BA_ERROR_INVALID_VALUE;
-  *done = it->second.state == State::Done || it->second.state == State::Failed;
-  if (*done) fill_result(s, it->second, res);
-  return BA_SUCCESS;
+  return ba_guarded([&]() -> int {
+    if (!s || !done) return BA_ERROR_INVALID_VALUE;
+    std::lock_guard<std::mutex> g(s->m);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return BA_ERROR_INVALID_VALUE;
+    *done = it->second.state == State::Done || it->second.state == State::Failed;
+    if (*done) fill_result(s, it->second, res);
+    return BA_SUCCESS;
+  });
 }
 
 extern "C" int ba_server_release(ba_server_t s, uint64_t ticket) {
-  if (!s) return BA_ERROR_INVALID_VALUE;
-  std::lock_guard<std::mutex> g(s->m);
-  auto it = s->reqs.find(ticket);
-  if (it == s->reqs.end() || (it->second.state != State::Done && it->second.state != State::Failed))
-    return BA_ERROR_INVALID_VALUE;
-  if (it->second.out_bytes) s->arena.release(it->second.out_off, it->second.out_bytes);
-  s->reqs.erase(it);
-  s->cv_work.notify_one();
-  return BA_SUCCESS;
+  return ba_guarded([&]() -> int {
+    if (!s) return BA_ERROR_INVALID_VALUE;
+    std::lock_guard<std::mutex> g(s->m);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end() || (it->second.state != State::Done && it->second.state != State::Failed))
+      return BA_ERROR_INVALID_VALUE;
+    if (it->second.out_bytes) s->arena.release(it->second.out_off, it->second.out_bytes);
+    s->reqs.erase(it);
+    s->cv_work.notify_one();
+    return BA_SUCCESS;
+  });
 }
 
 extern "C" int ba_server_stats(ba_server_t s, ba_serve_stats *st) {
-  if (!s || !st) return BA_ERROR_INVALID_VALUE;
-  std::lock_guard<std::mutex> g(s->m);
-  *st = ba_serve_stats{};
-  st->completed = s->completed;
-  st->failed = s->failed;
-  st->batches = s->batches;
-  st->output_elems = s->out_total;
-  st->budget_bytes = s->arena.size();
-  st->peak_bytes = s->arena.peak();
-  st->in_use_bytes = s->arena.used();
-  st->mean_batch_jobs = s->batches ? (double)s->batched_jobs / (double)s->batches : 0.0;
-  if (!s->latencies.empty()) {
-    std::vector<double> v = s->latencies;
-    auto q = [&](double f) {
-      const size_t k = std::min(v.size() - 1, (size_t)(f * (double)(v.size() - 1) + 0.5));
-      std::nth_element(v.begin(), v.begin() + k, v.end());
-      return v[k];
-    };
-    st->p50_ms = q(0.50);
-    st->p99_ms = q(0.99);
-    double sum = 0, mx = 0;
-    for (double x : s->latencies) sum += x, mx = std::max(mx, x);
-    st->mean_ms = sum / (double)s->latencies.size();
-    st->max_ms = mx;
-  }
-  if (s->have_first && s->completed) st->elapsed_s = ms_between(s->first_submit, s->last_done) / 1e3;
-  return BA_SUCCESS;
+  return ba_guarded([&]() -> int {
+    if (!s || !st) return BA_ERROR_INVALID_VALUE;
+    std::lock_guard<std::mutex> g(s->m);
+    *st = ba_serve_stats{};
+    st->completed = s->completed;
+    st->failed = s->failed;
+    st->batches = s->batches;
+    st->output_elems = s->out_total;
+    st->budget_bytes = s->arena.size();
+    st->peak_bytes = s->arena.peak();
+    st->in_use_bytes = s->arena.used();
+    st->mean_batch_jobs = s->batches ? (double)s->batched_jobs / (double)s->batches : 0.0;
+    if (!s->latencies.empty()) {
+      std::vector<double> v = s->latencies;
+      auto q = [&](double f) {
+        const size_t k = std::min(v.size() - 1, (size_t)(f * (double)(v.size() - 1) + 0.5));
+        std::nth_element(v.begin(), v.begin() + k, v.end());
+        return v[k];
+      };
+      st->p50_ms = q(0.50);
+      st->p99_ms = q(0.99);
+      double sum = 0, mx = 0;
+      for (double x : s->latencies) sum += x, mx = std::max(mx, x);
+      st->mean_ms = sum / (double)s->latencies.size();
+      st->max_ms = mx;
+    }
+    if (s->have_first && s->completed) st->elapsed_s = ms_between(s->first_submit, s->last_done) / 1e3;
+    return BA_SUCCESS;
+  });
 }
 
 extern "C" int ba_server_destroy(ba_server_t s) {
-  if (!s) return BA_ERROR_INVALID_VALUE;
-  if (s->worker.joinable()) {
-    {
-      std::lock_guard<std::mutex> g(s->m);
-      s->stop = true;
-      s->cv_work.notify_one();
+  return ba_guarded([&]() -> int {
+    if (!s) return BA_ERROR_INVALID_VALUE;
+    if (s->worker.joinable()) {
+      {
+        std::lock_guard<std::mutex> g(s->m);
+        s->stop = true;
+        s->cv_work.notify_one();
+      }
+      s->worker.join();  // every submitted request has completed
     }
-    s->worker.join();  // every submitted request has completed
-  }
-  for (Slot &sl : s->slots) {
-    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-    if (sl.stream) (void)hipStreamDestroy(sl.stream);
-    if (sl.done) (void)hipEventDestroy(sl.done);
-    if (sl.staging) (void)hipHostFree(sl.staging);
-    if (sl.desc) (void)hipFree(sl.desc);
-  }
-  if (s->base) (void)hipFree(s->base);
-  if (s->ctx) s->ctx->users.fetch_sub(1);
-  delete s;
-  return BA_SUCCESS;
+    for (Slot &sl : s->slots) {
+      if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+      if (sl.stream) (void)hipStreamDestroy(sl.stream);
+      if (sl.done) (void)hipEventDestroy(sl.done);
+      if (sl.staging) (void)hipHostFree(sl.staging);
+      if (sl.desc) (void)hipFree(sl.desc);
+    }
+    if (s->base) (void)hipFree(s->base);
+    if (s->ctx) s->ctx->users.fetch_sub(1);
+    delete s;
+    return BA_SUCCESS;
+  });
 }

@@ -142,74 +142,82 @@ bool context_ok(ba_context_t ctx) {
 
 extern "C" int ba_autotune(ba_context_t ctx, uint32_t log_n_min, uint32_t log_n_max, uint64_t elems_per_trial,
                            ba_stream_t stream) {
-  if (!ctx) return BA_ERROR_NOT_INITIALIZED;
-  if (!context_ok(ctx) || log_n_min < 1 || log_n_min > log_n_max || log_n_max > BA_MAX_LOG_N)
-    return BA_ERROR_INVALID_VALUE;
-  const uint64_t elems = std::max<uint64_t>(elems_per_trial, 1ull << log_n_max);
-  uint64_t *in = nullptr, *out = nullptr;
-  hipError_t err = hipMalloc(&in, elems * 8);
-  if (err == hipSuccess) err = hipMalloc(&out, elems * 8);
-  int rc = (int)err;
-  const auto saved = std::vector<BaTuned>(&ctx->tuned[0][0], &ctx->tuned[0][0] + 2 * (BA_MAX_LOG_N + 1));
-  if (rc == BA_SUCCESS) rc = ba_fill_uniform(in, elems, elems, 1, 0x7475u, stream);
-  {
-    Timer timer((hipStream_t)stream);
-    if (rc == BA_SUCCESS) rc = timer.init();
-    for (uint32_t log_n = log_n_min; rc == BA_SUCCESS && log_n <= log_n_max; log_n++)
-      for (int n2b = 1; rc == BA_SUCCESS && n2b >= 0; n2b--) rc = tune_class(ctx, log_n, n2b != 0, in, out, elems, timer);
-  }
-  if (rc != BA_SUCCESS) std::copy(saved.begin(), saved.end(), &ctx->tuned[0][0]);  // keep the old state
-  (void)hipStreamSynchronize((hipStream_t)stream);
-  if (in) (void)hipFree(in);
-  if (out) (void)hipFree(out);
-  return rc;
+  return ba_guarded([&]() -> int {
+    if (!ctx) return BA_ERROR_NOT_INITIALIZED;
+    if (!context_ok(ctx) || log_n_min < 1 || log_n_min > log_n_max || log_n_max > BA_MAX_LOG_N)
+      return BA_ERROR_INVALID_VALUE;
+    const uint64_t elems = std::max<uint64_t>(elems_per_trial, 1ull << log_n_max);
+    uint64_t *in = nullptr, *out = nullptr;
+    hipError_t err = hipMalloc(&in, elems * 8);
+    if (err == hipSuccess) err = hipMalloc(&out, elems * 8);
+    int rc = (int)err;
+    const auto saved = std::vector<BaTuned>(&ctx->tuned[0][0], &ctx->tuned[0][0] + 2 * (BA_MAX_LOG_N + 1));
+    if (rc == BA_SUCCESS) rc = ba_fill_uniform(in, elems, elems, 1, 0x7475u, stream);
+    {
+      Timer timer((hipStream_t)stream);
+      if (rc == BA_SUCCESS) rc = timer.init();
+      for (uint32_t log_n = log_n_min; rc == BA_SUCCESS && log_n <= log_n_max; log_n++)
+        for (int n2b = 1; rc == BA_SUCCESS && n2b >= 0; n2b--) rc = tune_class(ctx, log_n, n2b != 0, in, out, elems, timer);
+    }
+    if (rc != BA_SUCCESS) std::copy(saved.begin(), saved.end(), &ctx->tuned[0][0]);  // keep the old state
+    (void)hipStreamSynchronize((hipStream_t)stream);
+    if (in) (void)hipFree(in);
+    if (out) (void)hipFree(out);
+    return rc;
+  });
 }
 
 extern "C" int ba_tuning_get(ba_context_t ctx, uint32_t log_n, int bitrev_inputs, ba_tuned_plan *plan) {
-  if (!ctx) return BA_ERROR_NOT_INITIALIZED;
-  if (!plan || log_n < 1 || log_n > BA_MAX_LOG_N) return BA_ERROR_INVALID_VALUE;
-  const bool n2b = !bitrev_inputs;
-  const BaTuned &t = ctx->tuned[n2b][log_n];
-  const NttPlan p = t.valid ? t.plan : ba_plan_ntt(log_n, n2b);
-  *plan = ba_tuned_plan{};
-  plan->num_passes = (uint32_t)p.num_passes;
-  for (int i = 0; i < p.num_passes; i++) {
-    const int low = n2b ? p.num_passes - 1 - i : i;  // execution order -> low-bits-up order
-    plan->bits[low] = p.bits[i];
-    plan->reps[low] = t.valid ? t.reps[i] : 0;
-  }
-  return BA_SUCCESS;
+  return ba_guarded([&]() -> int {
+    if (!ctx) return BA_ERROR_NOT_INITIALIZED;
+    if (!plan || log_n < 1 || log_n > BA_MAX_LOG_N) return BA_ERROR_INVALID_VALUE;
+    const bool n2b = !bitrev_inputs;
+    const BaTuned &t = ctx->tuned[n2b][log_n];
+    const NttPlan p = t.valid ? t.plan : ba_plan_ntt(log_n, n2b);
+    *plan = ba_tuned_plan{};
+    plan->num_passes = (uint32_t)p.num_passes;
+    for (int i = 0; i < p.num_passes; i++) {
+      const int low = n2b ? p.num_passes - 1 - i : i;  // execution order -> low-bits-up order
+      plan->bits[low] = p.bits[i];
+      plan->reps[low] = t.valid ? t.reps[i] : 0;
+    }
+    return BA_SUCCESS;
+  });
 }
 
 extern "C" int ba_tuning_set(ba_context_t ctx, uint32_t log_n, int bitrev_inputs, const ba_tuned_plan *plan) {
-  if (!ctx) return BA_ERROR_NOT_INITIALIZED;
-  if (log_n < 1 || log_n > BA_MAX_LOG_N) return BA_ERROR_INVALID_VALUE;
-  const bool n2b = !bitrev_inputs;
-  if (!plan) {
-    ctx->tuned[n2b][log_n] = BaTuned{};
+  return ba_guarded([&]() -> int {
+    if (!ctx) return BA_ERROR_NOT_INITIALIZED;
+    if (log_n < 1 || log_n > BA_MAX_LOG_N) return BA_ERROR_INVALID_VALUE;
+    const bool n2b = !bitrev_inputs;
+    if (!plan) {
+      ctx->tuned[n2b][log_n] = BaTuned{};
+      return BA_SUCCESS;
+    }
+    if (plan->num_passes < 1 || plan->num_passes > 8) return BA_ERROR_INVALID_VALUE;
+    BaTuned e{};
+    if (!ba_plan_from_bits(log_n, n2b, plan->bits, (int)plan->num_passes, e.plan)) return BA_ERROR_INVALID_VALUE;
+    for (uint32_t i = 0; i < plan->num_passes; i++) {
+      if (plan->reps[i] > 8) return BA_ERROR_INVALID_VALUE;
+      const int exec = n2b ? (int)plan->num_passes - 1 - (int)i : (int)i;
+      e.reps[exec] = plan->reps[i];
+    }
+    e.valid = true;
+    ctx->tuned[n2b][log_n] = e;
     return BA_SUCCESS;
-  }
-  if (plan->num_passes < 1 || plan->num_passes > 8) return BA_ERROR_INVALID_VALUE;
-  BaTuned e{};
-  if (!ba_plan_from_bits(log_n, n2b, plan->bits, (int)plan->num_passes, e.plan)) return BA_ERROR_INVALID_VALUE;
-  for (uint32_t i = 0; i < plan->num_passes; i++) {
-    if (plan->reps[i] > 8) return BA_ERROR_INVALID_VALUE;
-    const int exec = n2b ? (int)plan->num_passes - 1 - (int)i : (int)i;
-    e.reps[exec] = plan->reps[i];
-  }
-  e.valid = true;
-  ctx->tuned[n2b][log_n] = e;
-  return BA_SUCCESS;
+  });
 }
 
 extern "C" int ba_tuning_candidates(uint32_t log_n, ba_tuned_plan *plans, uint32_t capacity, uint32_t *count) {
-  if (!count || log_n < 1 || log_n > BA_MAX_LOG_N || (capacity && !plans)) return BA_ERROR_INVALID_VALUE;
-  const auto splits = candidate_splits(log_n);
-  *count = (uint32_t)splits.size();
-  for (uint32_t i = 0; i < capacity && i < splits.size(); i++) {
-    plans[i] = ba_tuned_plan{};
-    plans[i].num_passes = (uint32_t)splits[i].size();
-    for (size_t k = 0; k < splits[i].size(); k++) plans[i].bits[k] = splits[i][k];
-  }
-  return BA_SUCCESS;
+  return ba_guarded([&]() -> int {
+    if (!count || log_n < 1 || log_n > BA_MAX_LOG_N || (capacity && !plans)) return BA_ERROR_INVALID_VALUE;
+    const auto splits = candidate_splits(log_n);
+    *count = (uint32_t)splits.size();
+    for (uint32_t i = 0; i < capacity && i < splits.size(); i++) {
+      plans[i] = ba_tuned_plan{};
+      plans[i].num_passes = (uint32_t)splits[i].size();
+      for (size_t k = 0; k < splits[i].size(); k++) plans[i].bits[k] = splits[i][k];
+    }
+    return BA_SUCCESS;
+  });
 }
