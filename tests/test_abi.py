@@ -170,3 +170,71 @@ def test_lde_huge_layout_check_allocates_nothing():
     assert _lde_rc(lib, (1 << 44) + 8, n, None, 1 << 44, n << log_b, n, log_n, cols, log_b) == 1
     # an output range past the end of the address space
     assert _lde_rc(lib, 1 << 40, n, None, (1 << 64) - (1 << 20), n << log_b, n, log_n, cols, log_b) == 1
+
+
+def _rust_fn_params(src, name):
+    """(generic parameters, [(param, type)], return type) of `pub fn name` in Rust source text."""
+    m = re.search(r"pub fn %s(<[^>]*>)?\((.*?)\)\s*->\s*([^{]+)\{" % name, src, flags=re.S)
+    assert m, name
+    params = []
+    depth, cur = 0, ""
+    for ch in m.group(2):
+        depth += ch in "<([" and 1 or (ch in ">)]" and -1 or 0)
+        if ch == "," and depth == 0:
+            params.append(cur)
+            cur = ""
+        else:
+            cur += ch
+    if cur.strip():
+        params.append(cur)
+    out = [tuple(x.strip() for x in p.split(":", 1)) for p in params if p.strip()]
+    return (m.group(1) or ""), out, " ".join(m.group(3).split())
+
+
+def test_rust_ntt_signatures_keep_boojum_cuda_call_sites():
+    """boojum-amd's ntt functions keep boojum_cuda's parameter names and order
+    (crates/boojum-cuda/src/ntt.rs:179-191, 294-308, 338-349), take era_cudart's
+    `&CudaStream` from the boojum-amd-cudart crate, return `CudaResult<()>`, and are generic over
+    the element type (`F: GoldilocksRepr`, implemented for boojum's GoldilocksField under the
+    `boojum` feature), so shivini's calls with `DeviceSlice<GoldilocksField>` compile unchanged
+    once the dependencies are renamed.  Not compiled here: there is no cargo."""
+    rust = os.path.join(ROOT, "zksync-crypto-gpu_amd", "rust")
+    ntt = open(os.path.join(rust, "boojum-amd", "src", "ntt.rs")).read()
+    lib = open(os.path.join(rust, "boojum-amd", "src", "lib.rs")).read()
+    want = {
+        "batch_ntt_internal": ["inputs_ptr_in", "outputs_ptr", "log_n", "num_ntts", "stride_between_input_arrays_in",
+                               "stride_between_output_arrays", "bitrev_inputs", "inverse", "log_extension_degree",
+                               "coset_index", "stream"],
+        "batch_ntt_out_of_place": ["inputs_matrix", "outputs_matrix", "log_n", "num_ntts", "inputs_offset",
+                                   "outputs_offset", "stride_between_input_arrays", "stride_between_output_arrays",
+                                   "bitrev_inputs", "inverse", "log_extension_degree", "coset_index", "stream"],
+        "batch_ntt_in_place": ["inputs_matrix", "log_n", "num_ntts", "inputs_offset", "stride_between_input_arrays",
+                               "bitrev_inputs", "inverse", "log_extension_degree", "coset_index", "stream"],
+    }
+    for name, params in want.items():
+        gen, got, ret = _rust_fn_params(ntt, name)
+        assert [p for p, _ in got] == params, name
+        assert gen == "<F: GoldilocksRepr>", name
+        assert ret == "CudaResult<()>", name
+        types = dict(got)
+        assert types["stream"] == "&CudaStream"
+        for p in ("log_n", "num_ntts", "log_extension_degree", "coset_index"):
+            assert types[p] == "u32", (name, p)
+        for p in ("bitrev_inputs", "inverse"):
+            assert types[p] == "bool", (name, p)
+    assert dict(_rust_fn_params(ntt, "batch_ntt_out_of_place")[1])["inputs_matrix"] == "&DeviceSlice<F>"
+    assert dict(_rust_fn_params(ntt, "batch_ntt_in_place")[1])["inputs_matrix"] == "&mut DeviceSlice<F>"
+    assert "use crate::cudart::stream::CudaStream;" in ntt and "use crate::cudart::slice::DeviceSlice;" in ntt
+    assert "unsafe impl GoldilocksRepr for boojum::field::goldilocks::GoldilocksField {}" in lib
+    # the era_cudart paths shivini's LDE code imports (crates/shivini/src/primitives/ntt.rs:1-30)
+    cud = os.path.join(rust, "boojum-amd-cudart", "src")
+    for mod, items in {"stream": ["pub struct CudaStream", "pub struct CudaStreamWaitEventFlags",
+                                  "pub const DEFAULT: CudaStream", "#[repr(transparent)]"],
+                       "slice": ["pub struct DeviceSlice<T>([T]);", "#[repr(transparent)]",
+                                 "pub unsafe fn from_mut_slice"],
+                       "result": ["pub type CudaResult<T> = Result<T, CudaError>;"],
+                       "event": ["pub struct CudaEvent"],
+                       "memory": ["pub fn memory_copy_async", "pub struct DeviceAllocation<T>"]}.items():
+        text = open(os.path.join(cud, mod + ".rs")).read()
+        for item in items:
+            assert item in text, (mod, item)

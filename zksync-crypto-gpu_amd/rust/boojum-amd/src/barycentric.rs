@@ -1,6 +1,8 @@
 //! `boojum_cuda::barycentric` (crates/boojum-cuda/src/barycentric.rs:83-300).
-use crate::runtime::{CudaStream, DeviceSlice};
-use crate::{check, sys, BaResult};
+use crate::cudart::result::CudaResult;
+use crate::cudart::slice::DeviceSlice;
+use crate::cudart::stream::CudaStream;
+use crate::{check, sys};
 
 /// Where the point X lies, and the evaluations' field, as the reference's Precompute/Eval impls.
 #[derive(Clone, Copy, Debug, PartialEq, Eq)]
@@ -15,7 +17,7 @@ pub enum Eval {
 /// count (base) or 2 * count (extension planes) elements, count a power of two.
 pub fn precompute_lagrange_coeffs(ext: bool, x: &DeviceSlice<u64>, common_factor_storage: &mut DeviceSlice<u64>,
                                   coset: u64, lagrange_coeffs: &mut DeviceSlice<u64>, bit_reverse: bool,
-                                  stream: &CudaStream) -> BaResult<()> {
+                                  stream: &CudaStream) -> CudaResult<()> {
     let width = if ext { 2 } else { 1 };
     assert!(x.len() >= width && common_factor_storage.len() >= width);
     let count = lagrange_coeffs.len() / width;
@@ -28,7 +30,7 @@ pub fn precompute_lagrange_coeffs(ext: bool, x: &DeviceSlice<u64>, common_factor
 }
 
 /// Elements of temp storage `batch_eval` needs (the reference's get_batch_eval_temp_storage_sizes).
-pub fn get_batch_eval_temp_storage_elems(kind: Eval, count: u32, num_polys: u32) -> BaResult<usize> {
+pub fn get_batch_eval_temp_storage_elems(kind: Eval, count: u32, num_polys: u32) -> CudaResult<usize> {
     let mut n = 0u64;
     check(unsafe { sys::ba_barycentric_batch_eval_temp_elems(kind as i32, count, num_polys, &mut n) },
           "batch_eval_temp")?;
@@ -39,7 +41,7 @@ pub fn get_batch_eval_temp_storage_elems(kind: Eval, count: u32, num_polys: u32)
 #[allow(clippy::too_many_arguments)]
 pub fn batch_eval(kind: Eval, batch_ys: &DeviceSlice<u64>, count: u32, num_polys: u32,
                   lagrange_coeffs: &DeviceSlice<u64>, temp: &mut DeviceSlice<u64>, evals: &mut DeviceSlice<u64>,
-                  stream: &CudaStream) -> BaResult<()> {
+                  stream: &CudaStream) -> CudaResult<()> {
     let width = if kind == Eval::BaseAtBase { 1 } else { 2 };
     assert_eq!(evals.len(), width * num_polys as usize);
     check(unsafe {

@@ -1,44 +1,56 @@
-//! Safe Rust mirror of the reference's NTT/LDE API over libboojum_amd.so.
+//! Safe Rust mirror of boojum_cuda's NTT/LDE API over libboojum_amd.so.
 //!
-//! Module and function names follow the reference crates so a caller switches by changing
-//! its `use` lines: `runtime` stands for era_cudart (crates/cudart/src/{stream,event,memory,
-//! slice}), `context`, `ntt`, `ops_complex`, `barycentric` and `poseidon2` for boojum_cuda's
-//! modules of the same names.  Field elements are `u64` (boojum's `GoldilocksField` is `#[repr(transparent)]`
-//! over `u64`, so a caller's `DeviceSlice<GoldilocksField>` is viewed as `DeviceSlice<u64>`).
-//! Status 1 from the library is where the reference asserts: these wrappers keep that panic,
-//! and return every other error as `Err`.
+//! Module and function names, argument lists and types follow boojum_cuda (crates/boojum-cuda/
+//! src/{context,ntt,ops_complex,barycentric,poseidon2}.rs) and era_cudart (through the
+//! boojum-amd-cudart crate), so a caller that renames its dependencies
+//! (`boojum_cuda = { package = "boojum-amd", features = ["boojum"] }`,
+//! `era_cudart = { package = "boojum-amd-cudart" }`) keeps its call sites: the field-element
+//! functions are generic over `GoldilocksRepr`, implemented for `u64` and, with the `boojum`
+//! feature, for boojum's `GoldilocksField`; the stream argument is era_cudart's `&CudaStream`;
+//! every call returns `CudaResult<()>`.  Status 1 from the library is where the reference
+//! asserts: these wrappers keep that panic and return every other status as `Err`.
 pub mod barycentric;
 pub mod batch;
 pub mod context;
 pub mod ntt;
 pub mod ops_complex;
 pub mod poseidon2;
-pub mod runtime;
 pub mod serve;
 
+pub use boojum_amd_cudart as cudart;
 pub use boojum_amd_sys as sys;
 
-/// A non-zero status of the library (a HIP error code or one of `sys::BA_ERROR_*`).
-#[derive(Clone, Copy, Debug, PartialEq, Eq)]
-pub struct BaError(pub i32);
+use cudart::result::{CudaError, CudaResult};
 
-impl core::fmt::Display for BaError {
-    fn fmt(&self, f: &mut core::fmt::Formatter<'_>) -> core::fmt::Result {
-        let name = unsafe { std::ffi::CStr::from_ptr(sys::ba_error_name(self.0)) };
-        write!(f, "boojum-amd status {} ({})", self.0, name.to_string_lossy())
-    }
+/// A Goldilocks field element stored as its `u64` representative: `#[repr(transparent)]` over
+/// `u64` (boojum's `GoldilocksField` is), so device slices of it are the library's u64 arrays.
+///
+/// # Safety
+/// Implementors are exactly 8 bytes, 8-aligned, with the value's u64 representative as their
+/// only field.
+pub unsafe trait GoldilocksRepr: Copy + 'static {}
+
+unsafe impl GoldilocksRepr for u64 {}
+
+#[cfg(feature = "boojum")]
+unsafe impl GoldilocksRepr for boojum::field::goldilocks::GoldilocksField {}
+
+/// Device pointer of a slice of field elements as the library's `u64` pointer.
+pub(crate) fn raw<F: GoldilocksRepr>(p: *const F) -> *const u64 {
+    assert!(core::mem::size_of::<F>() == 8 && core::mem::align_of::<F>() == 8);
+    p as *const u64
 }
-
-impl std::error::Error for BaError {}
-
-pub type BaResult<T> = Result<T, BaError>;
+pub(crate) fn raw_mut<F: GoldilocksRepr>(p: *mut F) -> *mut u64 {
+    assert!(core::mem::size_of::<F>() == 8 && core::mem::align_of::<F>() == 8);
+    p as *mut u64
+}
 
 /// Maps a status to a result; status 1 (invalid arguments) panics, as the reference's
 /// `assert!`s do (e.g. crates/boojum-cuda/src/ntt.rs:192-200).
-pub(crate) fn check(rc: i32, what: &str) -> BaResult<()> {
+pub(crate) fn check(rc: i32, what: &str) -> CudaResult<()> {
     match rc {
         0 => Ok(()),
         1 => panic!("{what}: invalid arguments"),
-        _ => Err(BaError(rc)),
+        _ => Err(CudaError(rc)),
     }
 }

@@ -4,8 +4,10 @@
 //! (`set_round_constants`, e.g. with `poseidon_goldilocks_params::ALL_ROUND_CONSTANTS_AS_FIELD_ELEMENTS`
 //! viewed as `u64`).
 use crate::context::Context;
-use crate::runtime::{CudaStream, DeviceSlice};
-use crate::{check, sys, BaResult};
+use crate::cudart::result::CudaResult;
+use crate::cudart::slice::DeviceSlice;
+use crate::cudart::stream::CudaStream;
+use crate::{check, sys};
 
 pub const RATE: usize = 8;
 pub const CAPACITY: usize = 4;
@@ -13,13 +15,13 @@ pub const ROUNDS: usize = 30;
 pub const WIDTH: usize = 12;
 
 /// Installs the 30 x 12 round constants (row-major by round) into `ctx`.
-pub fn set_round_constants(ctx: &Context, round_constants: &[u64; ROUNDS * WIDTH]) -> BaResult<()> {
+pub fn set_round_constants(ctx: &Context, round_constants: &[u64; ROUNDS * WIDTH]) -> CudaResult<()> {
     check(unsafe { sys::ba_poseidon2_gl_set_round_constants(ctx.raw(), round_constants.as_ptr()) },
           "poseidon2 set_round_constants")
 }
 
 pub fn compute_leaf_hashes(values: &DeviceSlice<u64>, results: &mut DeviceSlice<u64>, log_rows_per_hash: u32,
-                           load_intermediate: bool, store_intermediate: bool, stream: &CudaStream) -> BaResult<()> {
+                           load_intermediate: bool, store_intermediate: bool, stream: &CudaStream) -> CudaResult<()> {
     check(unsafe {
         sys::ba_poseidon2_gl_compute_leaf_hashes(values.as_ptr(), values.len(), results.as_mut_ptr(), results.len(),
                                                  log_rows_per_hash, load_intermediate as i32,
@@ -28,7 +30,7 @@ pub fn compute_leaf_hashes(values: &DeviceSlice<u64>, results: &mut DeviceSlice<
 }
 
 pub fn compute_node_hashes(values: &DeviceSlice<u64>, results: &mut DeviceSlice<u64>, stream: &CudaStream)
-                           -> BaResult<()> {
+                           -> CudaResult<()> {
     check(unsafe {
         sys::ba_poseidon2_gl_compute_node_hashes(values.as_ptr(), values.len(), results.as_mut_ptr(), results.len(),
                                                  stream.raw())
@@ -37,7 +39,7 @@ pub fn compute_node_hashes(values: &DeviceSlice<u64>, results: &mut DeviceSlice<
 
 pub fn build_merkle_tree_leaves(values: &DeviceSlice<u64>, results: &mut DeviceSlice<u64>, log_rows_per_hash: u32,
                                 load_intermediate: bool, store_intermediate: bool, stream: &CudaStream)
-                                -> BaResult<()> {
+                                -> CudaResult<()> {
     check(unsafe {
         sys::ba_poseidon2_gl_build_merkle_tree_leaves(values.as_ptr(), values.len(), results.as_mut_ptr(),
                                                       results.len(), log_rows_per_hash, load_intermediate as i32,
@@ -46,7 +48,7 @@ pub fn build_merkle_tree_leaves(values: &DeviceSlice<u64>, results: &mut DeviceS
 }
 
 pub fn build_merkle_tree_nodes(values: &DeviceSlice<u64>, results: &mut DeviceSlice<u64>, layers_count: u32,
-                               stream: &CudaStream) -> BaResult<()> {
+                               stream: &CudaStream) -> CudaResult<()> {
     check(unsafe {
         sys::ba_poseidon2_gl_build_merkle_tree_nodes(values.as_ptr(), values.len(), results.as_mut_ptr(),
                                                      results.len(), layers_count, stream.raw())
@@ -55,7 +57,7 @@ pub fn build_merkle_tree_nodes(values: &DeviceSlice<u64>, results: &mut DeviceSl
 
 /// Same argument order as `GpuTreeHasher::build_merkle_tree`.
 pub fn build_merkle_tree(values: &DeviceSlice<u64>, results: &mut DeviceSlice<u64>, log_rows_per_hash: u32,
-                         stream: &CudaStream, layers_count: u32) -> BaResult<()> {
+                         stream: &CudaStream, layers_count: u32) -> CudaResult<()> {
     check(unsafe {
         sys::ba_poseidon2_gl_build_merkle_tree(values.as_ptr(), values.len(), results.as_mut_ptr(), results.len(),
                                                log_rows_per_hash, layers_count, stream.raw())
