@@ -82,6 +82,99 @@ def test_field_ops(ba):
             t.free()
 
 
+def _run_op(ba, op, a, b):
+    da, db, do = ba.DeviceAllocation.from_host(a), ba.DeviceAllocation.from_host(b), ba.DeviceAllocation(a.size)
+    try:
+        ba.runtime.field_op(op, da, db, do, a.size)
+        return do.to_host()
+    finally:
+        for t in (da, db, do):
+            t.free()
+
+
+SHIFT_EDGES = [0, 1, 2, P - 2, P - 1, P, P + 1, 2**64 - 1, 2**32 - 1, 2**32, 2**63, 2**64 - 2**32,
+               2**63 - 1, 2**48, 2**16 - 1, 2**48 - 1, 2**64 - 2**48, 0xFFFFFFFF00000000,
+               0x0000FFFFFFFFFFFF, 0xFFFF000000000000, 0x00000000FFFF0000, 0x8000000080000000]
+
+
+def test_field_shifts(ba):
+    """x 2^E for every shift the NTT blocks use (multiples of 12 below 192) and the boundaries of
+    the three shift forms, on edge values of x (any u64, canonical or not) and random ones."""
+    exps = [12, 24, 36, 48, 60, 72, 84, 96, 108, 120, 132, 144, 156, 168, 180, 0, 1, 31, 32, 33, 63, 64,
+            65, 191]
+    rng = np.random.default_rng(11)
+    xs = [x for x in SHIFT_EDGES for _ in range(24)] + list(raw_u64(rng, 4096))
+    # small multiples of 2^(64-F) and 2^(96-F) make the shifted words wrap or vanish exactly
+    for f in (12, 24, 36, 48, 60):
+        xs += [k << (64 - f) for k in (1, 2, 3, 2**f - 1)] + [(k << (96 - f)) % 2**64 for k in (1, 5)]
+    sel = [i % 24 for i in range(24 * len(SHIFT_EDGES))] + list(rng.integers(0, 24, len(xs) - 24 * len(SHIFT_EDGES)))
+    # a factor 2^0 or 2^96 = -1 is no shift: those take canonical x, as everywhere they are used
+    xs = [x % P if exps[k] % 96 == 0 else x for x, k in zip(xs, sel)]
+    a = np.array(xs, dtype=np.uint64)
+    b = np.array(sel, dtype=np.uint64)
+    got = _run_op(ba, 5, a, b)
+    for i in range(a.size):
+        e = exps[int(b[i]) % 24]
+        assert int(got[i]) == (int(a[i]) << e) % P, ("pow2", hex(int(a[i])), e)
+
+
+def test_field_reductions(ba):
+    """The word reductions on chosen inputs that reach every carry/borrow case: lo + hl (2^32 - 1)
+    wrapping or not, the subtraction of hh (+ a borrow-in) going below zero or not, and results at
+    or above p, against (lo + hl 2^64 + (hh + bi) 2^96) mod p."""
+    rng = np.random.default_rng(13)
+    E = 2**32 - 1
+    los, words = [], []
+    for _ in range(64):
+        hl = int(rng.integers(0, 2**32))
+        hh = int(rng.integers(1, 2**31))
+        r = int(rng.integers(0, hh))
+        wrap = (2**64 - hl * E) % 2**64
+        for lo in (r, (wrap + r) % 2**64, (wrap + hh + r) % 2**64, (wrap + 2**64 - 1) % 2**64, 2**64 - 1 - r,
+                   P + r, int(rng.integers(0, 2**64, dtype=np.uint64))):
+            for hh_ in (hh, 0, 1):
+                for bi in (0, 1):
+                    los.append(lo)
+                    words.append(hl | (hh_ << 32) | (bi << 63))
+    # hl = 0: R = lo exactly; hh above it borrows
+    for lo in (0, 1, 5, P - 1, P, 2**64 - 1):
+        for hh_ in (0, 1, 7, 2**31 - 1):
+            for bi in (0, 1):
+                los.append(lo)
+                words.append((hh_ << 32) | (bi << 63))
+    a = np.array(los, dtype=np.uint64)
+    b = np.array(words, dtype=np.uint64)
+    got = _run_op(ba, 6, a, b)
+    for i in range(a.size):
+        lo, w = int(a[i]), int(b[i])
+        hl, hh, bi = w & 0xFFFFFFFF, (w >> 32) & 0x7FFFFFFF, w >> 63
+        want = (lo + hl * 2**64 + (hh + bi) * 2**96) % P
+        assert int(got[i]) == want, ("reduce3b", hex(lo), hl, hh, bi)
+    # reduce3: full 32-bit hh, no borrow-in
+    b3 = np.array([w & 0x7FFFFFFFFFFFFFFF | ((w & 1) << 63) for w in (int(x) for x in b)], dtype=np.uint64)
+    got = _run_op(ba, 8, a, b3)
+    for i in range(a.size):
+        lo, w = int(a[i]), int(b3[i])
+        hl, hh = w & 0xFFFFFFFF, w >> 32
+        assert int(got[i]) == (lo + hl * 2**64 + hh * 2**96) % P, ("reduce3", hex(lo), hl, hh)
+    got = _run_op(ba, 7, a, b)
+    for i in range(a.size):
+        lo, hl = int(a[i]), int(b[i]) & 0xFFFFFFFF
+        assert int(got[i]) == (lo + hl * 2**64) % P, ("reduce2", hex(lo), hl)
+
+
+def test_field_mul_edges(ba):
+    """Products whose 128-bit words make the reduction borrow or wrap: powers of two times
+    multiples of 2^48, 2^32 and 2^16 (x 2^96 = -x), and edge values against each other."""
+    vals = SHIFT_EDGES + [(k << s) % 2**64 for s in (16, 32, 48, 56) for k in (1, 3, 2**16 - 1)]
+    pairs = [(x, y) for x in vals for y in vals]
+    a = np.array([x for x, _ in pairs], dtype=np.uint64)
+    b = np.array([y for _, y in pairs], dtype=np.uint64)
+    got = _run_op(ba, 2, a, b)
+    for i in range(a.size):
+        assert int(got[i]) == (int(a[i]) * int(b[i])) % P, ("mul", hex(int(a[i])), hex(int(b[i])))
+
+
 def test_fill_uniform_matches_oracle(ba):
     rows, cols, seed = 1000, 3, 12345
     d = ba.DeviceAllocation(rows * cols)
