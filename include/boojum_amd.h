@@ -158,6 +158,15 @@ int ba_bit_reverse_in_place(uint64_t *values, uint64_t stride, uint32_t log_rows
 int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_stride, uint64_t *monomials,
                  uint64_t *out, uint64_t out_col_stride, uint64_t out_coset_stride, uint32_t log_n,
                  uint32_t num_cols, uint32_t log_blowup, ba_stream_t stream);
+/* ba_lde_batch with options.  BA_LDE_NATURAL_MONOMIALS: the monomials are left in natural order
+ * (coefficient i at position i), as shivini's into_monomials returns them after its bit-reverse
+ * epilogue (crates/shivini/src/data_structures/storage.rs:155, primitives/ntt.rs:572-585): one
+ * in-place bit reverse of the monomials on `stream` after every coset has read them.  Any other
+ * flag bit is an invalid value. */
+#define BA_LDE_NATURAL_MONOMIALS 1u
+int ba_lde_batch_ex(ba_context_t ctx, uint64_t *evals, uint64_t in_stride, uint64_t *monomials,
+                    uint64_t *out, uint64_t out_col_stride, uint64_t out_coset_stride, uint32_t log_n,
+                    uint32_t num_cols, uint32_t log_blowup, uint32_t flags, ba_stream_t stream);
 
 /* ---------------------------------------------------------------- powers, omega shifts
  * boojum_cuda::ops_complex (crates/boojum-cuda/src/ops_complex.rs:31-260): vectors of powers
@@ -279,7 +288,9 @@ int ba_tuning_candidates(uint32_t log_n, ba_tuned_plan *plans, uint32_t capacity
  * Clients submit from any thread; the server's scheduler thread admits requests in arrival
  * order while its device arena (one allocation of memory_budget_bytes) has room for the
  * request's working buffer (2^log_n x num_cols elements) and outputs (2^(log_n+log_blowup) x
- * num_cols), merges everything admitted since its last launch into one batch plan and enqueues
+ * num_cols) -- while the oldest queued request waits for memory, later requests that fit may pass
+ * it, at most max_bypass of them (default 32, ba_server_set_max_bypass; 0 = strict arrival
+ * order) --, merges everything admitted since its last launch into one batch plan and enqueues
  * it on a free slot stream (max_inflight_batches slots), so batches overlap on the device.
  * Results are the same as ba_lde_batch on the same evaluations: column k, coset c of the
  * output is at outputs + k * out_col_stride + c * 2^log_n (bit-reversed within the coset, the
@@ -316,6 +327,8 @@ typedef struct {
 typedef struct ba_server *ba_server_t;
 int ba_server_create(ba_context_t ctx, size_t memory_budget_bytes, uint32_t max_inflight_batches,
                      ba_server_t *server);
+/* how many later requests may be admitted ahead of a queued request waiting for memory */
+int ba_server_set_max_bypass(ba_server_t server, uint32_t max_bypass);
 int ba_server_submit(ba_server_t server, const ba_serve_request *request, uint64_t *ticket);
 /* blocks until the request completed or failed */
 int ba_server_wait(ba_server_t server, uint64_t ticket, ba_serve_result *result);

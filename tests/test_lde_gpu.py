@@ -78,6 +78,42 @@ def test_lde_layouts_and_scratch(ba, layout):
         assert np.array_equal(mono, gl_oracle.batch_ntt(ev, log_n, cols, n + 3, n, False, True))
 
 
+@pytest.mark.parametrize("keep_evals", [False, True])
+def test_lde_natural_monomials(ba, keep_evals):
+    """ba_lde_batch_ex with BA_LDE_NATURAL_MONOMIALS: the same coset evaluations, and the
+    monomials in natural order (shivini's into_monomials after its bit-reverse epilogue,
+    crates/shivini/src/data_structures/storage.rs:155), in a separate buffer or in place."""
+    for log_n in (1, 4, 9, 10, 13, 16):
+        cols, lb = 3, 2
+        n = 1 << log_n
+        log(f"lde natural monomials log_n={log_n} keep_evals={keep_evals}")
+        ev = raw_u64(np.random.default_rng(100 + log_n), cols * n)
+        d_ev = ba.DeviceAllocation.from_host(ev)
+        d_out = ba.DeviceAllocation((n << lb) * cols)
+        d_mono = ba.DeviceAllocation(n * cols) if keep_evals else None
+        try:
+            ba.lde.lde_batch(ba._test_ctx, d_ev, log_n, cols, lb, d_out, monomials=d_mono,
+                             natural_monomials=True)
+            ba.synchronize()
+            got = d_out.to_host()
+            mono = (d_mono if keep_evals else d_ev).to_host()
+        finally:
+            d_ev.free()
+            d_out.free()
+            if d_mono is not None:
+                d_mono.free()
+        assert np.array_equal(got, gl_oracle.lde(ev, log_n, cols, lb, nthreads=4)), log_n
+        mono_br = gl_oracle.batch_ntt(ev, log_n, cols, n, n, False, True)
+        perm = bitrev_perm(log_n)
+        want = np.concatenate([mono_br[k * n:(k + 1) * n][perm] for k in range(cols)])
+        assert np.array_equal(mono, want), log_n
+        # natural monomials are the inverse NTT in natural order: evaluating them at omega^i
+        # (a forward natural -> bit-reversed NTT) gives back the evaluations, bit-reversed
+        back = gl_oracle.batch_ntt(mono, log_n, cols, n, n, False, False)
+        ev_red = ev % np.uint64(P)
+        assert np.array_equal(np.concatenate([back[k * n:(k + 1) * n][perm] for k in range(cols)]), ev_red)
+
+
 def test_lde_golden(ba, golden):
     for log_n in range(1, 8):
         ev = golden[f"lde_in_{log_n}"]

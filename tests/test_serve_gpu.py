@@ -174,3 +174,110 @@ def test_config_shapes_alone_equals_mixed(ba):
         assert st["failed"] == 0 and st["in_use_bytes"] == 0
     finally:
         srv.destroy()
+
+
+def test_full_shapes_poisson_clients(ba):
+    """Requests at the serve configuration's full shapes: log_n 22 and 23, 128 and 256 columns,
+    blowup 8, submitted by three client threads at Poisson-timed arrivals while earlier ones run.
+    Each result equals the same request run alone (first, middle and last column compared) and
+    one column of each matches the oracle."""
+    import time
+    shapes = [(22, 16, 1), (23, 8, 2), (16, 128, 3), (12, 256, 3), (20, 12, 3), (14, 9, 2), (18, 40, 1),
+              (13, 200, 2), (21, 10, 1)]
+    srv = ba.serve.Server(ba._test_ctx, 24 << 30, max_inflight_batches=2)
+
+    def sample_cols(res, log_n, cols, lb):
+        out = srv.output(res, log_n, cols, lb)
+        m = 1 << (log_n + lb)
+        return {k: out.sub(k * m, m).to_host() for k in sorted({0, cols // 2, cols - 1})}
+
+    try:
+        alone = []
+        for i, (log_n, cols, lb) in enumerate(shapes):
+            log(f"alone log_n={log_n} cols={cols} blowup={1 << lb}")
+            res = srv.wait(srv.submit(log_n, cols, lb, seed=8100 + i))
+            assert res["status"] == 0
+            alone.append(sample_cols(res, log_n, cols, lb))
+            srv.release(res["ticket"])
+        tickets = [None] * len(shapes)
+        rng = np.random.default_rng(81)
+        delays = rng.exponential(0.004, len(shapes))
+
+        def client(idx):
+            for i in idx:
+                time.sleep(float(delays[i]))
+                log_n, cols, lb = shapes[i]
+                tickets[i] = srv.submit(log_n, cols, lb, seed=8100 + i)
+
+        threads = [threading.Thread(target=client, args=(list(range(c, len(shapes), 3)),)) for c in range(3)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        batches = set()
+        for i, (log_n, cols, lb) in enumerate(shapes):
+            res = srv.wait(tickets[i])
+            assert res["status"] == 0, res
+            batches.add(res["batch"])
+            got = sample_cols(res, log_n, cols, lb)
+            srv.release(tickets[i])
+            log(f"mixed log_n={log_n} cols={cols} blowup={1 << lb}: batch {res['batch']}, "
+                f"queue {res['queue_ms']:.1f} ms, latency {res['latency_ms']:.1f} ms")
+            for k in got:
+                assert np.array_equal(got[k], alone[i][k]), (log_n, cols, lb, k)
+            k = cols - 1
+            x = gl_oracle.fill_uniform(1 << log_n, column_seed(8100 + i, k))
+            assert np.array_equal(got[k], gl_oracle.lde(x, log_n, 1, lb, nthreads=8)), (log_n, cols, lb)
+        st = srv.stats()
+        log(f"stats {st}, batches {sorted(batches)}")
+        assert st["failed"] == 0 and st["in_use_bytes"] == 0
+    finally:
+        srv.destroy()
+
+
+def test_bypass_admits_small_requests_past_a_waiting_head(ba):
+    """A budget that holds one large request: while a second large request waits for memory (the
+    first one's outputs are still held), small requests submitted after it are admitted and
+    complete (max_bypass 32); with max_bypass 0 they stay queued behind it until the first large
+    result is released.  Every small result matches the oracle on one column."""
+    import time
+    big, small = (16, 16, 2), (10, 2, 1)
+    per = lambda s: (s[1] << s[0]) * 8 * (1 + (1 << s[2]))  # noqa: E731
+    budget = per(big) + 6 * per(small) + 8192
+
+    def poll_until(srv, t, seconds):
+        end = time.perf_counter() + seconds
+        while time.perf_counter() < end:
+            r = srv.poll(t)
+            if r is not None:
+                return r
+            time.sleep(0.002)
+        return None
+
+    for bypass in (32, 0):
+        srv = ba.serve.Server(ba._test_ctx, budget, max_inflight_batches=2)
+        srv.set_max_bypass(bypass)
+        try:
+            t_big1 = srv.submit(*big, seed=1)
+            assert srv.wait(t_big1)["status"] == 0  # its outputs stay held until released
+            t_big2 = srv.submit(*big, seed=2)  # waits for memory
+            t_small = [srv.submit(*small, seed=10 + i) for i in range(4)]
+            early = [poll_until(srv, t, 10.0 if bypass else 0.2) for t in t_small]
+            assert srv.poll(t_big2) is None  # still waiting for the first one's outputs
+            if bypass:
+                assert all(r is not None and r["status"] == 0 for r in early), early
+            else:
+                assert all(r is None for r in early), early
+            srv.release(t_big1)
+            assert srv.wait(t_big2)["status"] == 0
+            smalls = early if bypass else [srv.wait(t) for t in t_small]
+            for i, (r, t) in enumerate(zip(smalls, t_small)):
+                assert r["status"] == 0
+                got = srv.output(r, *small).sub(0, 1 << (small[0] + small[2])).to_host()
+                x = gl_oracle.fill_uniform(1 << small[0], column_seed(10 + i, 0))
+                assert np.array_equal(got, gl_oracle.lde(x, small[0], 1, small[2], nthreads=4))
+                srv.release(t)
+            srv.release(t_big2)
+            log(f"bypass {bypass}: small queue ms {[round(r['queue_ms'], 2) for r in smalls]}")
+        finally:
+            srv.destroy()

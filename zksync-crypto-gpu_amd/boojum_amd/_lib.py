@@ -85,6 +85,7 @@ SIGNATURES = {
     "ba_bit_reverse": [_vp, _u64, _vp, _u64, _u32, _u32, _vp],
     "ba_bit_reverse_in_place": [_vp, _u64, _u32, _u32, _vp],
     "ba_lde_batch": [_vp, _vp, _u64, _vp, _vp, _u64, _u64, _u32, _u32, _u32, _vp],
+    "ba_lde_batch_ex": [_vp, _vp, _u64, _vp, _vp, _u64, _u64, _u32, _u32, _u32, _u32, _vp],
     "ba_batch_plan_create": [_vp, ctypes.POINTER(Job), _u32, ctypes.POINTER(_vp)],
     "ba_batch_plan_launch": [_vp, _vp],
     "ba_batch_plan_stats": [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32), ctypes.POINTER(_u64)],
@@ -109,6 +110,7 @@ SIGNATURES = {
     "ba_poseidon2_gl_build_merkle_tree_nodes": [_vp, _sz, _vp, _sz, _u32, _vp],
     "ba_poseidon2_gl_build_merkle_tree": [_vp, _sz, _vp, _sz, _u32, _u32, _vp],
     "ba_server_create": [_vp, _sz, _u32, ctypes.POINTER(_vp)],
+    "ba_server_set_max_bypass": [_vp, _u32],
     "ba_server_submit": [_vp, ctypes.POINTER(ServeRequest), ctypes.POINTER(_u64)],
     "ba_server_wait": [_vp, _u64, ctypes.POINTER(ServeResult)],
     "ba_server_poll": [_vp, _u64, ctypes.POINTER(_i32), ctypes.POINTER(ServeResult)],

@@ -18,6 +18,11 @@ class Server:
         check(lib().ba_server_create(ctx.handle, memory_budget_bytes, max_inflight_batches,
                                      ctypes.byref(self.handle)), "server_create")
 
+    def set_max_bypass(self, max_bypass):
+        """Later requests that may be admitted ahead of a queued request waiting for memory
+        (0: strict arrival order)."""
+        check(lib().ba_server_set_max_bypass(self.handle, max_bypass), "server_set_max_bypass")
+
     def submit(self, log_n, num_cols, log_blowup, evals=None, seed=0, in_stride=0):
         """evals: a numpy uint64 array (copied in from host memory), a DeviceSlice (read in
         place), or None for seeded synthetic inputs.  Returns the ticket."""

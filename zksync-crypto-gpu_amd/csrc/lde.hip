@@ -129,11 +129,15 @@ static bool lde_serial() {
   return serial;
 }
 
-extern "C" int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_stride,
-                            uint64_t *monomials, uint64_t *out, uint64_t out_col_stride,
-                            uint64_t out_coset_stride, uint32_t log_n, uint32_t num_cols,
-                            uint32_t log_blowup, ba_stream_t stream) {
-  return ba_guarded([&]() -> int {
+// The LDE's launches on `stream` (cosets after the first forked onto the context's auxiliary
+// streams and joined back); with natural_monomials the monomials are bit-reversed in place once
+// every coset has read them.
+static int lde_run(ba_context_t ctx, uint64_t *evals, uint64_t in_stride, uint64_t *monomials,
+                   uint64_t *out, uint64_t out_col_stride, uint64_t out_coset_stride, uint32_t log_n,
+                   uint32_t num_cols, uint32_t log_blowup, bool natural_monomials, ba_stream_t stream) {
+  uint64_t *mono = monomials ? monomials : evals;
+  const uint64_t mono_stride = monomials ? (1ull << log_n) : in_stride;
+  const int rc = [&]() -> int {
     // arguments first (host-only checks), then the context
     int rc = ba_validate_lde(evals, in_stride, monomials, out, out_col_stride, out_coset_stride, log_n,
                              num_cols, log_blowup);
@@ -183,5 +187,28 @@ extern "C" int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_strid
       if (err != hipSuccess && !rc) rc = (int)err;
     }
     return rc;
+  }();
+  if (rc || !natural_monomials || num_cols == 0) return rc;
+  return ba_bit_reverse_in_place(mono, mono_stride, log_n, num_cols, stream);
+}
+
+extern "C" int ba_lde_batch(ba_context_t ctx, uint64_t *evals, uint64_t in_stride,
+                            uint64_t *monomials, uint64_t *out, uint64_t out_col_stride,
+                            uint64_t out_coset_stride, uint32_t log_n, uint32_t num_cols,
+                            uint32_t log_blowup, ba_stream_t stream) {
+  return ba_guarded([&]() -> int {
+    return lde_run(ctx, evals, in_stride, monomials, out, out_col_stride, out_coset_stride, log_n, num_cols,
+                   log_blowup, false, stream);
+  });
+}
+
+extern "C" int ba_lde_batch_ex(ba_context_t ctx, uint64_t *evals, uint64_t in_stride,
+                               uint64_t *monomials, uint64_t *out, uint64_t out_col_stride,
+                               uint64_t out_coset_stride, uint32_t log_n, uint32_t num_cols,
+                               uint32_t log_blowup, uint32_t flags, ba_stream_t stream) {
+  return ba_guarded([&]() -> int {
+    if (flags & ~(uint32_t)BA_LDE_NATURAL_MONOMIALS) return BA_ERROR_INVALID_VALUE;
+    return lde_run(ctx, evals, in_stride, monomials, out, out_col_stride, out_coset_stride, log_n, num_cols,
+                   log_blowup, (flags & BA_LDE_NATURAL_MONOMIALS) != 0, stream);
   });
 }

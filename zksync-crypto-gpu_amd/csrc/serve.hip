@@ -6,7 +6,9 @@
 // submit LDE requests from any thread; a scheduler thread owned by the server
 //   1. admits queued requests in arrival order while the device arena (one allocation of the
 //      memory budget, carved by a first-fit allocator) has room for the request's working
-//      buffer (evaluations -> monomials) and its outputs,
+//      buffer (evaluations -> monomials) and its outputs; while the oldest request waits for
+//      memory, later requests that fit may pass it, at most max_bypass of them (then the queue
+//      waits for the oldest), so a large request neither blocks small ones nor starves,
 //   2. turns everything admitted since the last launch into ONE batch plan (columns merged by
 //      transform, grouped pass launches) and enqueues it on a free slot stream, so a batch
 //      that arrives while others run overlaps with them on the device,
@@ -116,6 +118,9 @@ struct ba_server {
   std::mutex m;
   std::condition_variable cv_work, cv_done;
   std::deque<uint64_t> queued;
+  // requests admitted ahead of the current oldest queued request (head_ticket), and the bound
+  uint64_t head_ticket = 0;
+  uint32_t head_passed = 0, max_bypass = 32;
   std::vector<uint64_t> admitted;
   std::unordered_map<uint64_t, Request> reqs;
   uint64_t next_ticket = 1, batches = 0, completed = 0, failed = 0, out_total = 0, batched_jobs = 0;
@@ -296,21 +301,32 @@ void ba_server::loop() {
       else
         busy = true;
     }
-    // admission in arrival order: the head waits for memory, so large requests are not starved
-    while (!queued.empty()) {
-      Request &r = reqs.at(queued.front());
+    // admission in arrival order; while the head (oldest queued request) waits for memory, later
+    // requests that fit pass it, at most max_bypass per head, after which the queue waits for it
+    auto try_admit = [&](Request &r) {
       size_t w, o;
-      if (!arena.alloc(r.work_bytes, w)) break;
+      if (!arena.alloc(r.work_bytes, w)) return false;
       if (!arena.alloc(r.out_bytes, o)) {
         arena.release(w, r.work_bytes);
-        break;
+        return false;
       }
       r.work_off = w;
       r.out_off = o;
       r.state = State::Admitted;
       r.admit_t = Clock::now();
-      admitted.push_back(queued.front());
-      queued.pop_front();
+      return true;
+    };
+    for (auto it = queued.begin(); it != queued.end();) {
+      if (queued.front() != head_ticket) head_ticket = queued.front(), head_passed = 0;
+      const bool is_head = it == queued.begin();
+      if (!is_head && head_passed >= max_bypass) break;
+      if (try_admit(reqs.at(*it))) {
+        admitted.push_back(*it);
+        if (!is_head) head_passed++;
+        it = queued.erase(it);
+      } else {
+        ++it;
+      }
     }
     if (!admitted.empty()) {
       for (Slot &s : slots) {
@@ -372,6 +388,16 @@ extern "C" int ba_server_create(ba_context_t ctx, size_t memory_budget_bytes, ui
     }
     s->worker = std::thread([s] { s->loop(); });
     *out = s;
+    return BA_SUCCESS;
+  });
+}
+
+extern "C" int ba_server_set_max_bypass(ba_server_t s, uint32_t max_bypass) {
+  return ba_guarded([&]() -> int {
+    if (!s) return BA_ERROR_INVALID_VALUE;
+    std::lock_guard<std::mutex> g(s->m);
+    s->max_bypass = max_bypass;
+    s->cv_work.notify_one();
     return BA_SUCCESS;
   });
 }

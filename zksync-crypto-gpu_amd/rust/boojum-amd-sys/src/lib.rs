@@ -17,6 +17,7 @@ pub const BA_OMEGA_LOG_ORDER: u32 = 30;
 pub const BA_MAX_LOG_N: u32 = 28;
 pub const BA_JOB_NTT: u32 = 0;
 pub const BA_JOB_LDE: u32 = 1;
+pub const BA_LDE_NATURAL_MONOMIALS: u32 = 1;
 pub const BA_SERVE_INPUT_HOST: u32 = 0;
 pub const BA_SERVE_INPUT_DEVICE: u32 = 1;
 pub const BA_SERVE_INPUT_SYNTHETIC: u32 = 2;
@@ -188,6 +189,9 @@ extern "C" {
     pub fn ba_lde_batch(ctx: ba_context_t, evals: *mut u64, in_stride: u64, monomials: *mut u64, out: *mut u64,
                         out_col_stride: u64, out_coset_stride: u64, log_n: u32, num_cols: u32, log_blowup: u32,
                         stream: ba_stream_t) -> c_int;
+    pub fn ba_lde_batch_ex(ctx: ba_context_t, evals: *mut u64, in_stride: u64, monomials: *mut u64, out: *mut u64,
+                           out_col_stride: u64, out_coset_stride: u64, log_n: u32, num_cols: u32, log_blowup: u32,
+                           flags: u32, stream: ba_stream_t) -> c_int;
 
     // uneven batches, autotuning, serve mode (no reference counterpart)
     pub fn ba_batch_plan_create(ctx: ba_context_t, jobs: *const ba_job, num_jobs: u32,
@@ -203,6 +207,7 @@ extern "C" {
     pub fn ba_tuning_candidates(log_n: u32, plans: *mut ba_tuned_plan, capacity: u32, count: *mut u32) -> c_int;
     pub fn ba_server_create(ctx: ba_context_t, memory_budget_bytes: usize, max_inflight_batches: u32,
                             server: *mut ba_server_t) -> c_int;
+    pub fn ba_server_set_max_bypass(server: ba_server_t, max_bypass: u32) -> c_int;
     pub fn ba_server_submit(server: ba_server_t, request: *const ba_serve_request, ticket: *mut u64) -> c_int;
     pub fn ba_server_wait(server: ba_server_t, ticket: u64, result: *mut ba_serve_result) -> c_int;
     pub fn ba_server_poll(server: ba_server_t, ticket: u64, done: *mut c_int, result: *mut ba_serve_result) -> c_int;
