@@ -67,7 +67,13 @@ def cpu_baseline(log_n, log_blowup):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import gl_oracle
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    # one column per thread at a time, on the host cores this process may use, at most 16: a
+    # one-GPU box's share of its host is 16 cores, whatever the machine's total
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    threads = max(1, min(16, usable))
     cols = 2 * threads  # about 10-30 s of CPU work in all
     n = 1 << log_n
     ev = gl_oracle.fill_uniform(cols * n, 7)
@@ -81,10 +87,22 @@ def cpu_baseline(log_n, log_blowup):
     for _ in range(20):
         gl_oracle.batch_ntt(x16, 16, 1, 1 << 16, 1 << 16, False, False, nthreads=1)
     single_us = (time.perf_counter() - t1) / 20 * 1e6
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {
         "value": round(out_elems / dt / 1e6, 3),
         "unit": "Melem/s",
         "cores": threads,
+        "host_cpus": os.cpu_count(),
+        "usable_cpus": usable,
+        "cpu_model": model,
         "kind": "port",
         "sample": f"{cols} columns of the same 2^{log_n} blowup-{1 << log_blowup} LDE on {threads} threads, "
                   f"{dt:.2f} s wall ({dt * threads:.0f} thread-s)",
@@ -118,15 +136,13 @@ def run_single(ba, ctx, stream, log_n=16, calls=200):
             "us_per_call": round(us, 2), "melem_s": round(n / us, 1)}
 
 
-def run_uneven(ba, ctx, stream, seed=3, jobs_n=512, reps=5):
-    """Uneven batch: 512 single-column jobs, log_n uniform in 12..22, random forward/inverse
-    flag, 30% coset LDEs at blowup 2 or 4.  One batch plan (grouped launches) replayed as a
-    hipGraph, against one library call per job on the same stream."""
-    import ctypes
+def uneven_jobs(ba, seed=3, jobs_n=512):
+    """The uneven batch's jobs: single columns, log_n uniform in 12..22, random forward/inverse
+    flag, 30% coset LDEs at blowup 2 or 4; job i's input is ba_fill_uniform(seed 1000 + i).
+    Returns (plan jobs, per-job call tuples, device buffers, output elements)."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
-    lib = ba.lib()
     jobs, bufs, calls, out_elems = [], [], [], 0
     for i in range(jobs_n):
         log_n = int(rng.integers(12, 23))
@@ -147,6 +163,16 @@ def run_uneven(ba, ctx, stream, seed=3, jobs_n=512, reps=5):
             calls.append(("ntt", d_in, d_out, None, log_n, inverse))
             bufs += [d_in, d_out]
             out_elems += n
+    return jobs, calls, bufs, out_elems
+
+
+def run_uneven(ba, ctx, stream, seed=3, jobs_n=512, reps=5):
+    """Uneven batch (uneven_jobs): one batch plan (grouped launches) replayed as a hipGraph,
+    against one library call per job on the same stream."""
+    import ctypes
+
+    lib = ba.lib()
+    jobs, calls, bufs, out_elems = uneven_jobs(ba, seed, jobs_n)
     plan = ba.batch.BatchPlan(ctx, jobs)
     ba.check(lib.ba_stream_begin_capture(stream.handle), "begin_capture")
     plan.launch(stream)
@@ -413,11 +439,13 @@ def run_large(ba, ctx, stream, log_n=22, cols=256, lb=2, steps=3):
             "pass_kernels": kernels}
 
 
-def run_serve(ba, ctx, n_requests=240, rate_hz=40.0, seed=5, budget_frac=0.6, slots=2):
+def run_serve(ba, ctx, n_requests=240, rate_hz=40.0, seed=5, budget_frac=0.6, slots=2, max_bypass=32):
     """Serve mode: a Poisson stream of LDE requests (8..256 columns, log_n uniform in 12..23,
     blowup 2, 4 or 8, seeded synthetic inputs generated on the device) into a server whose
     arena is 60% of HBM.  One client thread submits at the arrival times; a reaper thread
-    releases each result as soon as it is done.  Latency is submit -> completion."""
+    releases each result as soon as it is done.  Latency is submit -> completion, split into
+    queue time (submit -> admitted: memory granted) and the rest (admitted -> done), and
+    reported per size class of output elements."""
     import threading
     import numpy as np
 
@@ -428,17 +456,21 @@ def run_serve(ba, ctx, n_requests=240, rate_hz=40.0, seed=5, budget_frac=0.6, sl
             for _ in range(n_requests)]
     arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, n_requests))
     srv = ba.serve.Server(ctx, budget, slots)
-    outstanding, lock, finished = [], threading.Lock(), threading.Event()
+    srv.set_max_bypass(max_bypass)
+    outstanding, lock, finished = {}, threading.Lock(), threading.Event()
+    results = {}
 
     def reaper():
         while True:
             with lock:
-                pending = list(outstanding)
-            for t in pending:
-                if srv.poll(t) is not None:
+                pending = list(outstanding.items())
+            for t, k in pending:
+                r = srv.poll(t)
+                if r is not None:
                     srv.release(t)
                     with lock:
-                        outstanding.remove(t)
+                        del outstanding[t]
+                        results[k] = r
             with lock:
                 if finished.is_set() and not outstanding:
                     return
@@ -447,22 +479,53 @@ def run_serve(ba, ctx, n_requests=240, rate_hz=40.0, seed=5, budget_frac=0.6, sl
     th = threading.Thread(target=reaper)
     th.start()
     t0 = time.perf_counter()
-    for (log_n, cols, lb), at in zip(reqs, arrivals):
+    for k, ((log_n, cols, lb), at) in enumerate(zip(reqs, arrivals)):
         delay = t0 + at - time.perf_counter()
         if delay > 0:
             time.sleep(delay)
         t = srv.submit(log_n, cols, lb, seed=int(rng.integers(0, 2**63)))
         with lock:
-            outstanding.append(t)
+            outstanding[t] = k
     finished.set()
     th.join()
     st = srv.stats()
     srv.destroy()
+
+    def pct(v, f):
+        v = sorted(v)
+        return v[min(len(v) - 1, int(f * (len(v) - 1) + 0.5))] if v else None
+
+    # size classes by output elements: small < 2^26 (512 MiB), medium < 2^30, large >= 2^30
+    classes = {"small (<2^26 out)": [], "medium (2^26..2^30)": [], "large (>=2^30)": []}
+    names = list(classes)
+    for k, (log_n, cols, lb) in enumerate(reqs):
+        r = results.get(k)
+        if r is None or r["status"] != 0:
+            continue
+        out = cols << (log_n + lb)
+        c = names[0] if out < (1 << 26) else names[1] if out < (1 << 30) else names[2]
+        classes[c].append((r["latency_ms"], r["queue_ms"], r["latency_ms"] - r["queue_ms"]))
+    per_class = {}
+    for c, v in classes.items():
+        if not v:
+            continue
+        lat, q, run = [x[0] for x in v], [x[1] for x in v], [x[2] for x in v]
+        per_class[c] = {"requests": len(v), "p50_ms": round(pct(lat, 0.5), 3), "p99_ms": round(pct(lat, 0.99), 3),
+                        "queue_p50_ms": round(pct(q, 0.5), 3), "queue_p99_ms": round(pct(q, 0.99), 3),
+                        "run_p50_ms": round(pct(run, 0.5), 3), "run_p99_ms": round(pct(run, 0.99), 3)}
+    allq = [r["queue_ms"] for r in results.values() if r["status"] == 0]
+    allrun = [r["latency_ms"] - r["queue_ms"] for r in results.values() if r["status"] == 0]
     return {"workload": "Poisson stream of coset-LDE requests: 8..256 columns, log_n U{12..23}, blowup "
                         "2/4/8, synthetic inputs filled on the device",
-            "requests": n_requests, "rate_hz": rate_hz, "slots": slots, "budget_bytes": st["budget_bytes"],
+            "requests": n_requests, "rate_hz": rate_hz, "slots": slots, "max_bypass": max_bypass,
+            "budget_bytes": st["budget_bytes"],
             "peak_bytes": st["peak_bytes"], "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
             "mean_ms": round(st["mean_ms"], 3), "max_ms": round(st["max_ms"], 3),
+            "queue_p50_ms": round(pct(allq, 0.5), 3) if allq else None,
+            "queue_p99_ms": round(pct(allq, 0.99), 3) if allq else None,
+            "run_p50_ms": round(pct(allrun, 0.5), 3) if allrun else None,
+            "run_p99_ms": round(pct(allrun, 0.99), 3) if allrun else None,
+            "per_size_class": per_class,
             "sustained_melem_s": round(st["output_elems"] / st["elapsed_s"] / 1e6, 1),
             "offered_melem_s": round(sum(c << (l + b) for l, c, b in reqs) / arrivals[-1] / 1e6, 1),
             "elapsed_s": round(st["elapsed_s"], 3), "batches": st["batches"],
@@ -557,13 +620,23 @@ def main():
     dom_avg_ms = dom_tot / dom_cnt
     # each pass launch reads and writes every element of the batch once: 16 B per element
     dom_bytes = 16 * n * cols
+    # the stages (index bits) the dominant pass transforms: its S template argument
+    import re
+    m = re.search(r"ntt_pass_(?:contig|strided)<(\d+),", dom_name)
+    dom_stages = int(m.group(1)) if m else 0
+    dom_bfly = dom_stages * n * cols // 2
     achieved = dom_bytes / (dom_avg_ms / 1e3) / 1e9
     traffic = None
     valu = None
+    prof_source = None
     if (log_n, cols, lb) == (20, 64, 1) and os.path.exists(TRAFFIC_FILE):
         prof = json.load(open(TRAFFIC_FILE))
         t = prof["traffic_bytes_per_launch"].get(dom_name)
         traffic = int(t) if t is not None else None
+        # the counters are not measurable inside this run: they come from the committed rocprofv3
+        # summary, tagged with the commit of the kernels it measured
+        prof_source = {"file": os.path.relpath(TRAFFIC_FILE, ROOT), "measured_commit": prof.get("measured_commit"),
+                       "counters": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE / SQ_INSTS_VALU"}
         v = prof.get("valu", {}).get(dom_name)
         bf = prof.get("butterfly_rate")
         if v is not None and bf is not None:
@@ -628,7 +701,15 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "avg_launch_ms": round(dom_avg_ms, 4),
-                     "alg_bytes_per_launch": dom_bytes},
+                     "alg_bytes_per_launch": dom_bytes,
+                     # which ceiling the kernel is nearer: its VALU rate against the measured VALU
+                     # ceiling, or its HBM rate against the HBM peak
+                     "limited_by": ("valu" if valu is not None and valu["frac"] > achieved / HBM_PEAK_GBS
+                                    else "hbm"),
+                     # useful work: radix-2 butterflies (n/2 per stage per column) per second
+                     "butterflies_per_s": round(dom_bfly / (dom_avg_ms / 1e3), 1),
+                     "element_stages_per_s": round(2 * dom_bfly / (dom_avg_ms / 1e3), 1),
+                     "counters_source": prof_source},
         "valu_roofline": valu,
         "per_kernel_ms": {k: round(v[0] / v[1], 4) for k, v in per_kernel.items()},
         "per_call": per_call,
@@ -658,7 +739,12 @@ def main():
     if not args.no_ops:
         line["ops"] = run_ops(ba, stream, d_ev, d_mono, log_n, cols)
     if not args.no_serve:
+        # the default policy (later requests that fit pass a head waiting for memory, at most 32 per
+        # head, two slots), then strict arrival order and four slots on the same request stream
         line["serve"] = run_serve(ba, ctx)
+        line["serve"]["variants"] = {
+            "strict_order_2_slots": run_serve(ba, ctx, max_bypass=0),
+            "bypass_4_slots": run_serve(ba, ctx, slots=4)}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(log_n, lb)
     ba.check(lib.ba_graph_destroy(graph), "graph_destroy")

@@ -29,6 +29,12 @@
 
 #include "gl_tables.h"
 
+// The contiguous pass loads the next tile into registers while it transforms the current one
+// (1: on; 0: each tile is loaded when its transform starts).
+#ifndef BA_CONTIG_PREFETCH
+#define BA_CONTIG_PREFETCH 1
+#endif
+
 namespace gl {
 namespace ntt {
 
@@ -500,10 +506,12 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
       tile0 < a.tile_count
           ? (u32)min((u64)a.reps, (a.tile_count - tile0 + (1ull << a.log_tile_stride) - 1) >> a.log_tile_stride)
           : 0;
-  if (reps) fetch(tile0);
+  constexpr bool PREFETCH = BA_CONTIG_PREFETCH;
+  if (PREFETCH && reps) fetch(tile0);
   for (u32 i = 0; i < reps; i++) {
     const u64 tile = tile0 + ((u64)i << a.log_tile_stride);
     const u64 g0 = tile << LOG_C;
+    if constexpr (!PREFETCH) fetch(tile);
     // keeps the 16 factors from being hoisted out of the loop (into 32 live registers)
     if constexpr (SCALE && AT_USE) asm volatile("" : "+v"(fbase));
     static_for<VALS>([&](auto k_) {
@@ -514,7 +522,7 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
       if (N2B && a.in_bitrev) slot = ((e >> S) << S) + bitrev32(e & ((1u << S) - 1), L);
       lds[lds_pad(slot)] = canon(nxt[k]);
     });
-    if (i + 1 < reps) fetch(tile + (1ull << a.log_tile_stride));
+    if (PREFETCH && i + 1 < reps) fetch(tile + (1ull << a.log_tile_stride));
     __syncthreads();
     u64 v[VALS];
     static_for<R>([&](auto r_) {
