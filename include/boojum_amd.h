@@ -221,6 +221,15 @@ int ba_barycentric_batch_eval_temp_elems(int kind, uint32_t count, uint32_t num_
 int ba_barycentric_batch_eval(int kind, const uint64_t *ys, uint32_t count, uint32_t num_polys,
                               const uint64_t *lagrange_coeffs, uint64_t *temp, uint64_t temp_elems,
                               uint64_t *evals, ba_stream_t stream);
+/* No counterpart in the reference.  ba_barycentric_batch_eval for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops, harnesses that forbid it): the
+ * partial sums go to a 4 MiB array the library holds on each GPU until the process ends, so the
+ * call needs no temp argument and allocates nothing.  Requests whose
+ * ba_barycentric_batch_eval_temp_elems exceeds 2^19 elements are an invalid value.  Every call on
+ * a GPU uses that GPU's one array: calls that overlap on one GPU (different streams, different
+ * threads) race. */
+int ba_barycentric_batch_eval_noalloc(int kind, const uint64_t *ys, uint32_t count, uint32_t num_polys,
+                                      const uint64_t *lagrange_coeffs, uint64_t *evals, ba_stream_t stream);
 
 /* ---------------------------------------------------------------- uneven batches
  * No reference counterpart (boojum-cuda runs one transform per call, ntt.rs:179-291): a list of

@@ -102,6 +102,7 @@ SIGNATURES = {
     "ba_barycentric_precompute_lagrange_coeffs": [_i32, _vp, _vp, _u64, _vp, _u32, _i32, _vp],
     "ba_barycentric_batch_eval_temp_elems": [_i32, _u32, _u32, ctypes.POINTER(_u64)],
     "ba_barycentric_batch_eval": [_i32, _vp, _u32, _u32, _vp, _vp, _u64, _vp, _vp],
+    "ba_barycentric_batch_eval_noalloc": [_i32, _vp, _u32, _u32, _vp, _vp, _vp],
     "ba_diag_butterfly_rate": [_vp, _vp, _u32, _u32, _vp],
     "ba_poseidon2_gl_set_round_constants": [_vp, _vp],
     "ba_poseidon2_gl_compute_leaf_hashes": [_vp, _sz, _vp, _sz, _u32, _i32, _i32, _vp],

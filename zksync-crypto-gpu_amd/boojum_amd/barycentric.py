@@ -30,3 +30,13 @@ def batch_eval_temp_elems(kind, count, num_polys):
 def batch_eval(kind, ys, count, num_polys, lagrange_coeffs, temp, evals, stream=None):
     check(lib().ba_barycentric_batch_eval(kind, ys.ptr, count, num_polys, lagrange_coeffs.ptr, temp.ptr, temp.len,
                                           evals.ptr, _h(stream)), "batch_eval")
+
+
+NOALLOC_TEMP_ELEMS = 1 << 19  # the library's per-GPU array behind batch_eval_noalloc
+
+
+def batch_eval_noalloc(kind, ys, count, num_polys, lagrange_coeffs, evals, stream=None):
+    """batch_eval with the partial sums in the library's per-GPU array: no temp argument, no
+    allocation; calls that overlap on one GPU race."""
+    check(lib().ba_barycentric_batch_eval_noalloc(kind, ys.ptr, count, num_polys, lagrange_coeffs.ptr, evals.ptr,
+                                                  _h(stream)), "batch_eval_noalloc")

@@ -19,6 +19,8 @@
 // Both are HBM-bound for base-field data (8 B per element read once) and VALU-bound for
 // extension-field products.
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "api_internal.h"
 
@@ -312,4 +314,53 @@ extern "C" int ba_barycentric_batch_eval(int kind, const uint64_t *ys, uint32_t 
       break;
   }
   return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------- batch evaluation, no allocation
+// For callers that cannot allocate device memory (code under stream capture, tight serving loops):
+// the partial sums live in this file-scope array instead of a caller workspace.  Every GPU the
+// library runs on holds its own copy, at its own address, until the process ends.  The partial
+// kernel writes every element the final kernel reads, so nothing has to start at zero; calls that
+// overlap on one GPU share the array and race.
+constexpr uint64_t kStaticTempElems = 1ull << 19;  // 4 MiB
+static __device__ __align__(256) uint64_t scratch[kStaticTempElems];
+
+namespace {
+std::mutex g_scratch_mutex;
+std::vector<uint64_t *> g_scratch_ptr;  // per device: the device address of `scratch`
+
+// The device address of this GPU's copy of `scratch`, looked up once per device (with that device
+// current) and kept; the array's name is a host-side stand-in and is never passed to a kernel.
+int scratch_for_current_device(uint64_t **ptr) {
+  int device = -1;
+  hipError_t err = hipGetDevice(&device);
+  if (err != hipSuccess) return (int)err;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (err != hipSuccess) return (int)err;
+    if (!p) return BA_ERROR_NOT_INITIALIZED;
+    g_scratch_ptr[device] = (uint64_t *)p;
+  }
+  *ptr = g_scratch_ptr[device];
+  return BA_SUCCESS;
+}
+}  // namespace
+
+extern "C" int ba_barycentric_batch_eval_noalloc(int kind, const uint64_t *ys, uint32_t count, uint32_t num_polys,
+                                                 const uint64_t *lagrange_coeffs, uint64_t *evals,
+                                                 ba_stream_t stream) {
+  return ba_guarded([&]() -> int {
+    uint64_t need = 0;
+    if (int rc = ba_barycentric_batch_eval_temp_elems(kind, count, num_polys, &need)) return rc;
+    if (num_polys == 0) return BA_SUCCESS;
+    if (need > kStaticTempElems) return BA_ERROR_INVALID_VALUE;  // beyond the array's 2^19 elements
+    if (!ys || !lagrange_coeffs || !evals) return BA_ERROR_INVALID_VALUE;
+    uint64_t *temp = nullptr;
+    if (int rc = scratch_for_current_device(&temp)) return rc;
+    return ba_barycentric_batch_eval(kind, ys, count, num_polys, lagrange_coeffs, temp, kStaticTempElems, evals,
+                                     stream);
+  });
 }

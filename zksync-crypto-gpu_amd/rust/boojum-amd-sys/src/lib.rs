@@ -184,6 +184,10 @@ extern "C" {
     pub fn ba_barycentric_batch_eval(kind: c_int, ys: *const u64, count: u32, num_polys: u32,
                                      lagrange_coeffs: *const u64, temp: *mut u64, temp_elems: u64,
                                      evals: *mut u64, stream: ba_stream_t) -> c_int;
+    // no reference counterpart: batch_eval for callers that cannot allocate (per-GPU array inside)
+    pub fn ba_barycentric_batch_eval_noalloc(kind: c_int, ys: *const u64, count: u32, num_polys: u32,
+                                             lagrange_coeffs: *const u64, evals: *mut u64,
+                                             stream: ba_stream_t) -> c_int;
 
     // shivini's LDE composition in one call (crates/shivini/src/data_structures/storage.rs:154-260)
     pub fn ba_lde_batch(ctx: ba_context_t, evals: *mut u64, in_stride: u64, monomials: *mut u64, out: *mut u64,
