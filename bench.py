@@ -657,6 +657,31 @@ def main():
                     "model_peak": VALU_PEAK_WINST / 1e9,
                     "model_frac": round(achieved_w / VALU_PEAK_WINST, 4)}
 
+    # the same graph with the contiguous pass pinned at 3 and at 4 waves per SIMD (the built-in
+    # choice above takes 4 at this size); steps timed as above
+    contig_waves = {}
+    if not args.tune_headline and (log_n, cols, lb) == (20, 64, 1):
+        for waves in (3, 4):
+            for br in (False, True):
+                bits, _ = ba.tune.get_plan(ctx, log_n, br)
+                ba.tune.set_plan(ctx, log_n, br, bits, None, contig_waves=waves)
+            ba.check(lib.ba_stream_begin_capture(stream.handle), "begin_capture")
+            step_eager()
+            g2 = ctypes.c_void_p(None)
+            ba.check(lib.ba_stream_end_capture(stream.handle, ctypes.byref(g2)), "end_capture")
+            for _ in range(10):
+                ba.check(lib.ba_graph_launch(g2, stream.handle), "graph_launch")
+            v0, v1 = ba.CudaEvent(), ba.CudaEvent()
+            v0.record(stream)
+            for _ in range(args.steps):
+                ba.check(lib.ba_graph_launch(g2, stream.handle), "graph_launch")
+            v1.record(stream)
+            v1.synchronize()
+            contig_waves[str(waves)] = round(v0.elapsed_ms(v1) / args.steps, 4)
+            ba.check(lib.ba_graph_destroy(g2), "graph_destroy")
+        for br in (False, True):
+            ba.tune.set_plan(ctx, log_n, br, None)
+
     per_call = None
     if not args.no_per_call:
         # one ba_lde_batch call per column (6 launches each), eager, as a caller without batching
@@ -711,6 +736,7 @@ def main():
                      "element_stages_per_s": round(2 * dom_bfly / (dom_avg_ms / 1e3), 1),
                      "counters_source": prof_source},
         "valu_roofline": valu,
+        "ms_per_step_contig_waves": contig_waves or None,
         "per_kernel_ms": {k: round(v[0] / v[1], 4) for k, v in per_kernel.items()},
         "per_call": per_call,
         "tuning": tuning,

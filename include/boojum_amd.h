@@ -268,10 +268,13 @@ int ba_batch_plan_destroy(ba_batch_plan_t plan);
  * class is (log_n, input order); its plan is the split of the log_n index bits into passes --
  * bits[0] for the contiguous pass over the lowest bits, then the strided passes upwards -- and,
  * per pass in the same order, the columns each workgroup repeats over (reps, 1..8; 0 = built-in
- * choice).  Valid splits: log_n <= 12 one pass of log_n bits; log_n > 12 a contiguous pass of
- * 9..12 bits and strided passes of 4..8 bits (at most 8 passes).
+ * choice), and the waves per SIMD of a 9..12-bit contiguous pass (contig_waves: 3 = the variant
+ * that loads the next tile while transforming, 4 = the 128-register variant without that
+ * prefetch, 0 = built-in choice: 3 below 2048 workgroups, else 4).  Valid splits: log_n <= 12
+ * one pass of log_n bits; log_n > 12 a contiguous pass of 9..12 bits and strided passes of 4..8
+ * bits (at most 8 passes).
  *
- * ba_autotune times every valid split, then each pass's reps, for log_n in [log_n_min,
+ * ba_autotune times every valid split, then each pass's reps, then contig_waves, for log_n in [log_n_min,
  * log_n_max] and both input orders, on batches of max(elems_per_trial, 2^log_n) elements of
  * scratch memory it allocates and frees, and keeps the fastest on `ctx`; every later transform
  * (NTT, LDE, batch plan) on `ctx` uses it.  Results are bit-identical for every plan.  Tuning
@@ -280,6 +283,7 @@ typedef struct {
   uint32_t num_passes;
   uint32_t bits[8];
   uint32_t reps[8];
+  uint32_t contig_waves; /* 0, 3 or 4 */
 } ba_tuned_plan;
 int ba_autotune(ba_context_t ctx, uint32_t log_n_min, uint32_t log_n_max, uint64_t elems_per_trial,
                 ba_stream_t stream);

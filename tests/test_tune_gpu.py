@@ -44,10 +44,11 @@ def test_default_plans(ba):
     assert t.get_plan(ba._test_ctx, 28) == ([12, 8, 8], [0, 0, 0])
 
 
-@pytest.mark.parametrize("log_n", [13, 14, 16, 20, 21])
+@pytest.mark.parametrize("log_n", [12, 13, 14, 16, 20, 21])
 def test_every_split_bit_exact(ba, log_n):
-    """Each split the autotuner can choose, with reps cycling through 0/1/2/4/8, in both input
-    orders, forward with a coset and inverse, against the oracle."""
+    """Each split the autotuner can choose, with reps cycling through 0/1/2/4/8 and the
+    contiguous pass's waves per SIMD through built-in/3/4, in both input orders, forward with a
+    coset and inverse, against the oracle."""
     cols = 3
     n = 1 << log_n
     rng = np.random.default_rng(log_n)
@@ -59,12 +60,14 @@ def test_every_split_bit_exact(ba, log_n):
     reps_cycle = [0, 1, 2, 4, 8]
     for k, bits in enumerate(splits):
         reps = [reps_cycle[(k + i) % 5] for i in range(len(bits))]
-        log(f"split log_n={log_n} bits={bits} reps={reps}")
-        for m in modes:
-            ba.tune.set_plan(ba._test_ctx, log_n, m[0], bits, reps)
-            assert ba.tune.get_plan(ba._test_ctx, log_n, m[0]) == (bits, reps)
-            got = run_ntt(ba, x, log_n, cols, *m)
-            assert np.array_equal(got, wants[m]), (bits, reps, m)
+        for waves in ([0, 3, 4] if k < 2 else [(0, 3, 4)[k % 3]]):
+            log(f"split log_n={log_n} bits={bits} reps={reps} contig_waves={waves}")
+            for m in modes:
+                ba.tune.set_plan(ba._test_ctx, log_n, m[0], bits, reps, contig_waves=waves)
+                assert ba.tune.get_plan(ba._test_ctx, log_n, m[0]) == (bits, reps)
+                assert ba.tune.get_contig_waves(ba._test_ctx, log_n, m[0]) == waves
+                got = run_ntt(ba, x, log_n, cols, *m)
+                assert np.array_equal(got, wants[m]), (bits, reps, waves, m)
     for br in (False, True):
         ba.tune.set_plan(ba._test_ctx, log_n, br, None)
 
@@ -105,6 +108,9 @@ def test_set_plan_rejects_invalid(ba):
             t.set_plan(ba._test_ctx, log_n, False, bits)
     with pytest.raises(ba.BoojumAmdError):
         t.set_plan(ba._test_ctx, 20, False, [12, 8], [9, 0])  # reps above 8
+    for waves in (1, 2, 5, 8):  # the contiguous pass runs at 3 or 4 waves per SIMD
+        with pytest.raises(ba.BoojumAmdError):
+            t.set_plan(ba._test_ctx, 20, False, [12, 8], [0, 0], contig_waves=waves)
     for log_n in (0, 29):
         with pytest.raises(ba.BoojumAmdError):
             t.get_plan(ba._test_ctx, log_n)

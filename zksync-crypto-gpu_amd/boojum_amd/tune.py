@@ -1,4 +1,5 @@
-"""Autotuning of pass splits and columns per workgroup (include/boojum_amd.h: ba_autotune,
+"""Autotuning of pass splits, columns per workgroup and the contiguous pass's waves per SIMD
+(include/boojum_amd.h: ba_autotune,
 ba_tuning_get, ba_tuning_set).  No reference counterpart: boojum-cuda's kernel shapes are fixed
 tables (crates/boojum-cuda/src/ntt.rs:57-176)."""
 import ctypes
@@ -22,7 +23,14 @@ def get_plan(ctx, log_n, bitrev_inputs=False):
     return list(p.bits[:n]), list(p.reps[:n])
 
 
-def set_plan(ctx, log_n, bitrev_inputs, bits, reps=None):
+def get_contig_waves(ctx, log_n, bitrev_inputs=False):
+    """Waves per SIMD of the size class's 9..12-bit contiguous pass (3, 4; 0 = built-in choice)."""
+    p = TunedPlan()
+    check(lib().ba_tuning_get(ctx.handle, log_n, int(bitrev_inputs), ctypes.byref(p)), "tuning_get")
+    return p.contig_waves
+
+
+def set_plan(ctx, log_n, bitrev_inputs, bits, reps=None, contig_waves=0):
     """Sets the plan of a size class; bits=None restores the built-in plan."""
     if bits is None:
         check(lib().ba_tuning_set(ctx.handle, log_n, int(bitrev_inputs), None), "tuning_set")
@@ -33,6 +41,7 @@ def set_plan(ctx, log_n, bitrev_inputs, bits, reps=None):
         p.bits[i] = b
     for i, r in enumerate((reps or [])[:8]):
         p.reps[i] = r
+    p.contig_waves = contig_waves
     check(lib().ba_tuning_set(ctx.handle, log_n, int(bitrev_inputs), ctypes.byref(p)), "tuning_set")
 
 

@@ -38,6 +38,7 @@ struct BaTuned {
   bool valid;
   NttPlan plan;
   uint32_t reps[8];
+  uint32_t contig_waves;  // waves per SIMD of the 9..12-bit contiguous pass: 3, 4, or 0 = built-in
 };
 
 struct ba_context {

@@ -28,7 +28,8 @@ __host__ __device__ __forceinline__ u64 canon(u64 x) { return x >= P ? x - P : x
 // at the full-width VALU rate (4.2-4.9 cycles per wave instruction per SIMD: carry-writing adds,
 // v_mad_u64_u32, 64-bit compares and shifts alike), while the SALU mask logic runs beside the
 // VALU; so the arithmetic below is written for the fewest VALU instructions, with the carry
-// cases resolved by SALU mask logic and carry-in adds rather than by selects.
+// cases resolved by mask logic on the carries (plain C on the 64-bit lane masks, which the
+// compiler emits as SALU instructions) and carry-in adds rather than by selects.
 namespace isa {
 
 __device__ __forceinline__ u32 lo(u64 x) { return (u32)x; }
@@ -103,34 +104,18 @@ __device__ __forceinline__ u64 ge_p(u64 x) {
   asm("v_cmp_ge_u64_e64 %0, %1, %2" : "=s"(m) : "v"(x), "s"(P));
   return m;
 }
-__device__ __forceinline__ u64 andn2(u64 a, u64 b) {
-  u64 r;
-  asm("s_andn2_b64 %0, %1, %2" : "=s"(r) : "s"(a), "s"(b) : "scc");
-  return r;
-}
-__device__ __forceinline__ u64 or_(u64 a, u64 b) {
-  u64 r;
-  asm("s_or_b64 %0, %1, %2" : "=s"(r) : "s"(a), "s"(b) : "scc");
-  return r;
-}
-__device__ __forceinline__ u64 xor_(u64 a, u64 b) {
-  u64 r;
-  asm("s_xor_b64 %0, %1, %2" : "=s"(r) : "s"(a), "s"(b) : "scc");
-  return r;
-}
-
 // x + EPS in the lanes of m (x + 2^32 - 1: low word - 1, high word + 1 unless the low word
 // borrowed); wraps mod 2^64 where the sum overflows
 __device__ __forceinline__ u64 plus_eps(u32 x0, u32 x1, u64 m) {
   u64 k;
   const u32 r0 = subb0(x0, m, k);
-  return pack(r0, addc0(x1, andn2(m, k)));
+  return pack(r0, addc0(x1, m & ~k));
 }
 // x - EPS in the lanes of m (low word + 1, high word - 1 unless the low word carried)
 __device__ __forceinline__ u64 minus_eps(u32 x0, u32 x1, u64 m) {
   u64 k;
   const u32 r0 = addc0(x0, m, k);
-  return pack(r0, subb0(x1, andn2(m, k)));
+  return pack(r0, subb0(x1, m & ~k));
 }
 
 }  // namespace isa
@@ -142,7 +127,7 @@ __device__ __forceinline__ u64 add(u64 a, u64 b) {
   u64 c, c2;
   const u32 s0 = add_co(lo(a), lo(b), c);
   const u32 s1 = addc(hi(a), hi(b), c, c2);
-  return plus_eps(s0, s1, or_(c2, ge_p(pack(s0, s1))));
+  return plus_eps(s0, s1, c2 | ge_p(pack(s0, s1)));
 }
 
 // a < p, b < p: canonical a - b.  d = a - b with borrow; a borrow selects d - EPS (= a - b + p,
@@ -179,8 +164,8 @@ __device__ __forceinline__ u64 reduce3b(u64 lo_, u32 hl, u32 hh, u64 bi) {
   const u32 d0 = subb(lo(R), hh, bi, b1);
   const u32 d1 = subb0(hi(R), b1, b);
   const u64 ge = ge_p(pack(d0, d1));
-  const u64 m_sub = andn2(b, c);
-  const u64 m_add = or_(andn2(c, b), andn2(ge, xor_(c, b)));
+  const u64 m_sub = b & ~c;
+  const u64 m_add = (c & ~b) | (ge & ~(c ^ b));
   const u64 e = minus_eps(d0, d1, m_sub);
   return plus_eps(lo(e), hi(e), m_add);
 }
@@ -191,8 +176,8 @@ __device__ __forceinline__ u64 reduce3(u64 lo_, u32 hl, u32 hh) {
   const u32 d0 = sub_co(lo(R), hh, b1);
   const u32 d1 = subb0(hi(R), b1, b);
   const u64 ge = ge_p(pack(d0, d1));
-  const u64 m_sub = andn2(b, c);
-  const u64 m_add = or_(andn2(c, b), andn2(ge, xor_(c, b)));
+  const u64 m_sub = b & ~c;
+  const u64 m_add = (c & ~b) | (ge & ~(c ^ b));
   const u64 e = minus_eps(d0, d1, m_sub);
   return plus_eps(lo(e), hi(e), m_add);
 }
@@ -203,7 +188,7 @@ __device__ __forceinline__ u64 reduce2(u64 lo_, u32 hl) {
   using namespace isa;
   u64 c;
   const u64 R = mad_eps(hl, lo_, c);
-  return plus_eps(lo(R), hi(R), or_(c, ge_p(R)));
+  return plus_eps(lo(R), hi(R), c | ge_p(R));
 }
 
 // Reduces lo + hi * 2^64 (any 128-bit value) to a canonical residue.

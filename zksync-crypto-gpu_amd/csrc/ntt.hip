@@ -83,17 +83,21 @@ using namespace gl::ntt;
 
 // scale: 0 none, 1 one factor for every element, 2 per-element coset factors (the strided
 // kernels fuse 1 and 2 alike)
+// waves: the 9..12-bit contiguous pass's variant (3 or 4 waves per SIMD)
 BaKernels pick_kernel(bool contig, uint32_t S, bool n2b, bool inverse, bool top, int scale,
-                      bool whole, bool gather) {
-  if (contig)
-    return S <= 8 ? ba_pick_contig_lo(S, n2b, inverse, scale, whole)
-                  : ba_pick_contig_hi(S, n2b, inverse, scale, whole);
+                      bool whole, bool gather, uint32_t waves) {
+  if (contig) {
+    if (S <= 8) return ba_pick_contig_lo(S, n2b, inverse, scale, whole);
+    return waves == 3 ? ba_pick_contig_hi3(S, n2b, inverse, scale, whole)
+                      : ba_pick_contig_hi(S, n2b, inverse, scale, whole);
+  }
   return ba_pick_strided(S, n2b, inverse, top, scale != 0, gather);
 }
 
 // kernel names as rocprofv3 prints them, for launch tracing; the table is built once, by the
 // first caller (a function-local static initialiser, so concurrent first calls are safe)
 struct KernelNames {
+  // [contig][S][n2b][inverse][top or whole][scale][gather or 4-waves]
   char n[2][13][2][2][2][3][2][120];
   KernelNames() {
     static const char *tf[2] = {"false", "true"};
@@ -104,8 +108,8 @@ struct KernelNames {
             for (int sc = 0; sc < 3; sc++)
               for (int ga = 0; ga < 2; ga++) {
                 snprintf(n[1][s][d][i][tp][sc][ga], sizeof(n[1][s][d][i][tp][sc][ga]),
-                         "void gl::ntt::ntt_pass_contig<%d, %s, %s, %d, %s>(gl::ntt::PassArgs)", s, tf[d],
-                         tf[i], sc, tf[tp]);
+                         "void gl::ntt::ntt_pass_contig<%d, %s, %s, %d, %s, %d>(gl::ntt::PassArgs)", s, tf[d],
+                         tf[i], sc, tf[tp], ga ? 4 : 3);
                 snprintf(n[0][s][d][i][tp][sc][ga], sizeof(n[0][s][d][i][tp][sc][ga]),
                          "void gl::ntt::ntt_pass_strided<%d, %s, %s, %s, %s, %s>(gl::ntt::PassArgs)", s,
                          tf[d], tf[i], tf[tp], tf[sc != 0], tf[ga]);
@@ -114,9 +118,9 @@ struct KernelNames {
 };
 
 const char *kernel_name(bool contig, uint32_t S, bool n2b, bool inverse, bool top, int scale,
-                        bool gather) {
+                        bool gather, uint32_t waves) {
   static const KernelNames names;
-  return names.n[contig][S][n2b][inverse][top][scale][gather];
+  return names.n[contig][S][n2b][inverse][top][scale][contig ? waves == 4 : gather];
 }
 
 constexpr uint64_t kMaxGrid = 0x7FFFFFFFull;
@@ -213,8 +217,14 @@ int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint3
       const bool top = plan.lo_bit[p] + S == log_n;
       const int scale = a.scale == SCALE_NONE ? 0 : (a.scale_coset ? 2 : 1);
       const bool gather = !contig && a.in_bitrev;
-      l.kernels = pick_kernel(contig, S, n2b, inverse, top, scale, big, gather);
-      l.name = kernel_name(contig, S, n2b, inverse, contig ? big : top, scale, gather);
+      // the 9..12-bit contiguous pass: as tuned, else 3 waves per SIMD (with the next-tile
+      // prefetch) below 2048 workgroups -- few tiles, where a workgroup's own instruction stream
+      // sets the time -- and 4 waves above
+      uint32_t waves = 3;
+      if (contig && S >= 9)
+        waves = tuning.valid && tuning.contig_waves ? tuning.contig_waves : (l.grid < 2048 ? 3 : 4);
+      l.kernels = pick_kernel(contig, S, n2b, inverse, top, scale, big, gather, waves);
+      l.name = kernel_name(contig, S, n2b, inverse, contig ? big : top, scale, gather, waves);
       l.pass = p;
       l.col0 = c0;
       if (!l.kernels.single) return BA_ERROR_INVALID_VALUE;

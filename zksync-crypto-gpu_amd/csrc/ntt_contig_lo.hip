@@ -3,9 +3,10 @@
 
 using namespace gl::ntt;
 
+// short columns (a tile holds whole columns) and 4..8-bit passes: the three-waves variant
 template <int S, bool N2B, bool INV, int SC, bool WHOLE>
 static BaKernels pair() {
-  return {ntt_pass_contig<S, N2B, INV, SC, WHOLE>, ntt_pass_contig_group<S, N2B, INV, SC, WHOLE>};
+  return {ntt_pass_contig<S, N2B, INV, SC, WHOLE, 3>, ntt_pass_contig_group<S, N2B, INV, SC, WHOLE, 3>};
 }
 
 // columns of >= 4096 elements use S = 12, or S = 9..11 when log_n is 13..15; shorter

@@ -18,7 +18,8 @@ struct BaKernels {
 // contiguous pass over bits [0, S), 4 <= S <= 8 / 9 <= S <= 12; scale: 0 none, 1 one factor
 // for every element, 2 per-element coset factors
 BaKernels ba_pick_contig_lo(uint32_t S, bool n2b, bool inverse, int scale, bool whole);
-BaKernels ba_pick_contig_hi(uint32_t S, bool n2b, bool inverse, int scale, bool whole);
+BaKernels ba_pick_contig_hi(uint32_t S, bool n2b, bool inverse, int scale, bool whole);   // 4 waves/SIMD
+BaKernels ba_pick_contig_hi3(uint32_t S, bool n2b, bool inverse, int scale, bool whole);  // 3, prefetch
 // strided pass, 4 <= S <= 8; top: the pass holds the top bits; scale: fused (pre/post) scale;
 // gather: the n2b top pass reads bit-reversed columns
 BaKernels ba_pick_strided(uint32_t S, bool n2b, bool inverse, bool top, bool scale, bool gather);

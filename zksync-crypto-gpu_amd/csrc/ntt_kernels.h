@@ -27,13 +27,9 @@
 
 #include <utility>
 
+#include "gl_phased.h"
 #include "gl_tables.h"
 
-// The contiguous pass loads the next tile into registers while it transforms the current one
-// (1: on; 0: each tile is loaded when its transform starts).
-#ifndef BA_CONTIG_PREFETCH
-#define BA_CONTIG_PREFETCH 1
-#endif
 
 namespace gl {
 namespace ntt {
@@ -127,15 +123,82 @@ __device__ __forceinline__ void dif_group(u64 (&v)[VALS]) {
   });
 }
 
-// One radix-2^K round over all 16 registers: register m = i * 2^(4-K) + m_lo, i active.
+// The 8 butterflies of stage t of one radix-2^K round over all 16 registers (register
+// m = i * 2^(4-K) + lo, i the K-bit local index), numbered b = lo 2^(K-1) + q half + jj: the
+// registers they pair and their factor 2^E (the same flows as dit_group / dif_group).
 template <int K, bool N2B, bool INV>
-__device__ __forceinline__ void local_round(u64 (&v)[VALS]) {
+struct RoundStage {
+  static constexpr int STRIDE = 1 << (4 - K);
+  static constexpr int half(int t) { return N2B ? 1 << (K - 1 - t) : 1 << t; }
+  static constexpr int j(int t, int b) {
+    const int r = b & ((1 << (K - 1)) - 1);
+    return (r / half(t)) * 2 * half(t) + r % half(t);
+  }
+  static constexpr int ia(int t, int b) { return (b >> (K - 1)) + j(t, b) * STRIDE; }
+  static constexpr int ib(int t, int b) { return (b >> (K - 1)) + (j(t, b) + half(t)) * STRIDE; }
+  static constexpr int E(int t, int b) {
+    const int q = (b & ((1 << (K - 1)) - 1)) / half(t);
+    return N2B ? local_exp(t + 1, cbitrev(q, t), INV) : local_exp(K - t, cbitrev(q, K - t - 1), INV);
+  }
+  static constexpr int phases(int t, int b) {
+    const int e = E(t, b), f = e % 96;
+    return ph::ADDSUB_PHASES + ph::shift_phases(f);
+  }
+};
+
+// One radix-2^K round over all 16 registers, one butterfly after another (register
+// m = i * 2^(4-K) + m_lo, i active).
+template <int K, bool N2B, bool INV>
+__device__ __forceinline__ void local_round_seq(u64 (&v)[VALS]) {
   static_for<(1 << (4 - K))>([&](auto lo_) {
     constexpr int lo = decltype(lo_)::value;
     if constexpr (N2B)
       dit_group<K, (1 << (4 - K)), lo, INV>(v);
     else
       dif_group<K, (1 << (4 - K)), lo, INV>(v);
+  });
+}
+
+// The same round stage by stage; each stage's 8 independent butterflies run ph::ILP at a time,
+// phase-interleaved.
+template <int K, bool N2B, bool INV>
+__device__ __forceinline__ void local_round_ilv(u64 (&v)[VALS]) {
+  using St = RoundStage<K, N2B, INV>;
+  static_assert(8 % ph::ILP == 0, "interleave width must divide a stage's 8 butterflies");
+  static_for<K>([&](auto t_) {
+    constexpr int t = decltype(t_)::value;
+    static_for<8 / ph::ILP>([&](auto g_) {
+      constexpr int g = decltype(g_)::value;
+      ph::Bfly st[ph::ILP];
+      static_for<ph::ILP>([&](auto j_) {
+        constexpr int b = g * ph::ILP + decltype(j_)::value;
+        st[decltype(j_)::value].a = v[St::ia(t, b)];
+        st[decltype(j_)::value].b = v[St::ib(t, b)];
+      });
+      constexpr int MAXP = [] {
+        int m = 0;
+        for (int k = 0; k < ph::ILP; k++) m = St::phases(t, g * ph::ILP + k) > m ? St::phases(t, g * ph::ILP + k) : m;
+        return m;
+      }();
+      static_for<MAXP>([&](auto p_) {
+        constexpr int p = decltype(p_)::value;
+        static_for<ph::ILP>([&](auto j_) {
+          constexpr int b = g * ph::ILP + decltype(j_)::value;
+          constexpr int e = St::E(t, b);
+          if constexpr (p < St::phases(t, b)) {
+            if constexpr (N2B)
+              ph::dit_phase<e, p>(st[decltype(j_)::value]);
+            else
+              ph::dif_phase<e, p>(st[decltype(j_)::value]);
+          }
+        });
+      });
+      static_for<ph::ILP>([&](auto j_) {
+        constexpr int b = g * ph::ILP + decltype(j_)::value;
+        v[St::ia(t, b)] = st[decltype(j_)::value].a;
+        v[St::ib(t, b)] = st[decltype(j_)::value].b;
+      });
+    });
   });
 }
 
@@ -150,14 +213,41 @@ __device__ __forceinline__ void twiddle_powers(u64 (&pw)[VALS], u64 w) {
   });
 }
 
-// v[m] *= W^(m >> (4-K))
-template <int K>
+// v[m] *= W^(m >> (4-K)) for the registers with a factor other than W^0; ILV: interleaved
+template <int K, bool ILV>
 __device__ __forceinline__ void apply_twiddles(u64 (&v)[VALS], const u64 (&pw)[VALS]) {
-  static_for<VALS>([&](auto m_) {
-    constexpr int m = decltype(m_)::value;
-    constexpr int i = m >> (4 - K);
-    if constexpr (i > 0) v[m] = mul(v[m], pw[i]);
-  });
+  constexpr int FIRST = 1 << (4 - K);
+  if constexpr (ILV) {
+    ph::mul_batch<VALS - FIRST>([&](auto i) -> u64 & { return v[FIRST + decltype(i)::value]; },
+                                [&](auto i) { return pw[(FIRST + decltype(i)::value) >> (4 - K)]; });
+  } else {
+    static_for<VALS>([&](auto m_) {
+      constexpr int m = decltype(m_)::value;
+      if constexpr (m >= FIRST) v[m] = mul(v[m], pw[m >> (4 - K)]);
+    });
+  }
+}
+
+// v[m] *= f(m) for all 16 registers; ILV: interleaved
+template <bool ILV, typename F>
+__device__ __forceinline__ void mul_all(u64 (&v)[VALS], F &&f) {
+  if constexpr (ILV) {
+    ph::mul_batch<VALS>([&](auto i) -> u64 & { return v[decltype(i)::value]; },
+                        [&](auto i) { return f(decltype(i)::value); });
+  } else {
+    static_for<VALS>([&](auto m_) {
+      constexpr int m = decltype(m_)::value;
+      v[m] = mul(v[m], f(m));
+    });
+  }
+}
+
+template <int K, bool N2B, bool INV, bool ILV>
+__device__ __forceinline__ void local_round(u64 (&v)[VALS]) {
+  if constexpr (ILV)
+    local_round_ilv<K, N2B, INV>(v);
+  else
+    local_round_seq<K, N2B, INV>(v);
 }
 
 enum : u32 { SCALE_NONE = 0, SCALE_PRE = 1, SCALE_POST = 2 };
@@ -251,11 +341,13 @@ __device__ __forceinline__ u64 round_twiddle(const PassArgs &a, u32 gidx, u32 P,
   return e ? pow_w(a.tab, e, inverse) : 1;
 }
 
-template <int K, bool N2B, bool INV>
+// ILV: the round's products and butterflies phase-interleaved (the strided pass, where that
+// measured faster) or one after another (the contiguous pass, where it measured slower)
+template <int K, bool N2B, bool INV, bool ILV>
 __device__ __forceinline__ void do_round(u64 (&v)[VALS], const u64 (&pw)[VALS], bool twiddle) {
-  if (N2B && twiddle) apply_twiddles<K>(v, pw);
-  local_round<K, N2B, INV>(v);
-  if (!N2B && twiddle) apply_twiddles<K>(v, pw);
+  if (N2B && twiddle) apply_twiddles<K, ILV>(v, pw);
+  local_round<K, N2B, INV, ILV>(v);
+  if (!N2B && twiddle) apply_twiddles<K, ILV>(v, pw);
 }
 
 // Loads/stores address a column as a uniform base plus a 32-bit per-thread byte offset
@@ -358,12 +450,7 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
       });
     }
     if (col + 1 < col_end) fetch(col + 1);
-    if constexpr (N2B && SCALE) {
-      static_for<VALS>([&](auto m_) {
-        constexpr int m = decltype(m_)::value;
-        v[m] = mul(v[m], fac[m]);
-      });
-    }
+    if constexpr (N2B && SCALE) mul_all<true>(v, [&](int m) { return fac[m]; });
     static_for<R>([&](auto r_) {
       constexpr int r = decltype(r_)::value;
       if constexpr (r > 0) {
@@ -381,14 +468,9 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
         });
         __syncthreads();
       }
-      do_round<Sh::K(r), N2B, INV>(v, tw[r], !(TOP && r == Sh::TOP));
+      do_round<Sh::K(r), N2B, INV, true>(v, tw[r], !(TOP && r == Sh::TOP));
     });
-    if constexpr (!N2B && SCALE) {
-      static_for<VALS>([&](auto m_) {
-        constexpr int m = decltype(m_)::value;
-        v[m] = mul(v[m], fac[m]);
-      });
-    }
+    if constexpr (!N2B && SCALE) mul_all<true>(v, [&](int m) { return fac[m]; });
     static_for<VALS>([&](auto m_) {
       constexpr int m = decltype(m_)::value;
       u64 *bm = out + ((u64)m << (WBL + P0));
@@ -401,7 +483,10 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
 // 1 one factor for every element (n^-1 of an inverse without coset), 2 per-element coset factors.
 // WHOLE: columns of >= 4096 elements (a tile is one contiguous block of one column); otherwise
 // a tile holds 4096 / 2^S whole columns of 2^S elements.
-template <int S, bool N2B, bool INV, int SC, bool WHOLE>
+// PREFETCH: the next tile is loaded into registers while the current one is transformed (the
+// three-waves-per-SIMD variant); otherwise each tile is loaded when its transform starts (the
+// four-waves variant, which fits 128 registers).
+template <int S, bool N2B, bool INV, int SC, bool WHOLE, bool PREFETCH>
 __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
   constexpr bool SCALE = SC == 2;
   const u64 uniform = a.scale_tab[0];
@@ -475,11 +560,16 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
     else
       scale_factors<PERM>(fac, a, j0, INV);
   }
-  auto factor = [&](int m) -> u64 {
-    if constexpr (AT_USE)
-      return mul(fbase, a.scale_tab[PERM ? cbitrev(m, 4) : m]);
-    else
-      return fac[m];
+  // v[m] *= the scale factor of register m (F(j0) T[pi(m)], formed here when AT_USE), or the
+  // one factor for all elements
+  auto scale_all = [&](u64(&v)[VALS]) {
+    if constexpr (!SCALE) {
+      mul_all<false>(v, [&](int) { return uniform; });
+    } else if constexpr (AT_USE) {
+      mul_all<false>(v, [&](int m) { return mul(fbase, a.scale_tab[PERM ? cbitrev(m, 4) : m]); });
+    } else {
+      mul_all<false>(v, [&](int m) { return fac[m]; });
+    }
   };
 
   // the next tile's elements are loaded while the current one is transformed
@@ -506,7 +596,6 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
       tile0 < a.tile_count
           ? (u32)min((u64)a.reps, (a.tile_count - tile0 + (1ull << a.log_tile_stride) - 1) >> a.log_tile_stride)
           : 0;
-  constexpr bool PREFETCH = BA_CONTIG_PREFETCH;
   if (PREFETCH && reps) fetch(tile0);
   for (u32 i = 0; i < reps; i++) {
     const u64 tile = tile0 + ((u64)i << a.log_tile_stride);
@@ -540,12 +629,7 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
         v[m] = lds[slot(m)];
       });
       __syncthreads();
-      if (SC != 0 && !INV && r == 0) {
-        static_for<VALS>([&](auto m_) {
-          constexpr int m = decltype(m_)::value;
-          if constexpr (SCALE) v[m] = mul(v[m], factor(m)); else v[m] = mul(v[m], uniform);
-        });
-      }
+      if (SC != 0 && !INV && r == 0) scale_all(v);
       if constexpr (WB >= 4) {
         constexpr int K = Sh::K(r), QB = S - 4 - WB;
         const u32 base = ((c << QB) | (q >> WB)) << K;
@@ -554,23 +638,15 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
           constexpr int i = decltype(i_)::value;
           pw[i] = ltw[r][base + i];
         });
-        do_round<K, N2B, INV>(v, pw, !(top && r == Sh::TOP));
+        do_round<K, N2B, INV, false>(v, pw, !(top && r == Sh::TOP));
       } else if constexpr (FOLD && r == R - 1) {
         constexpr int K = Sh::K(r);
-        static_for<VALS>([&](auto m_) {
-          constexpr int m = decltype(m_)::value;
-          v[m] = mul(v[m], tw[r][m >> (4 - K)]);
-        });
-        local_round<K, N2B, INV>(v);
+        mul_all<false>(v, [&](int m) { return tw[r][m >> (4 - K)]; });
+        local_round<K, N2B, INV, false>(v);
       } else {
-        do_round<Sh::K(r), N2B, INV>(v, tw[r], !(top && r == Sh::TOP));
+        do_round<Sh::K(r), N2B, INV, false>(v, tw[r], !(top && r == Sh::TOP));
       }
-      if (SC != 0 && INV && r == R - 1 && !FOLD) {
-        static_for<VALS>([&](auto m_) {
-          constexpr int m = decltype(m_)::value;
-          if constexpr (SCALE) v[m] = mul(v[m], factor(m)); else v[m] = mul(v[m], uniform);
-        });
-      }
+      if (SC != 0 && INV && r == R - 1 && !FOLD) scale_all(v);
       static_for<VALS>([&](auto m_) {
         constexpr int m = decltype(m_)::value;
         lds[slot(m)] = v[m];
@@ -667,21 +743,21 @@ __global__ __launch_bounds__(THREADS) void ntt_pass_strided_group(const PassArgs
   const u32 j = group_of(wg_start, njobs, blockIdx.x);
   strided_body<S, N2B, INV, TOP, SCALE, GATHER>(jobs[j], blockIdx.x - wg_start[j]);
 }
-// Waves per SIMD the contiguous pass is compiled for: 3 fits its registers once the upper
-// rounds' twiddles live in LDS; the per-element-scaled variants over short columns keep 16 more
-// values per thread and run at 2.
-#ifndef BA_CONTIG_WAVES
-#define BA_CONTIG_WAVES 3
-#endif
-template <int S, bool N2B, bool INV, int SC, bool WHOLE>
-__global__ __launch_bounds__(THREADS, SC == 2 && !WHOLE ? 2 : BA_CONTIG_WAVES) void ntt_pass_contig(PassArgs a) {
-  contig_body<S, N2B, INV, SC, WHOLE>(a, blockIdx.x);
+// W: waves per SIMD the contiguous pass is compiled for.  4: no prefetch, 128 registers (fastest
+// per launch over many tiles); 3: with the next-tile prefetch (fewer instructions per tile, better
+// when few workgroups run, and it leaves LDS room for a concurrent launch).  The per-element-
+// scaled variants over short columns keep 16 more values per thread and run at 2 either way.
+// The autotuner picks W per size class (ba_tuned_plan.contig_waves).
+template <int S, bool N2B, bool INV, int SC, bool WHOLE, int W>
+__global__ __launch_bounds__(THREADS, SC == 2 && !WHOLE ? 2 : W) void ntt_pass_contig(PassArgs a) {
+  contig_body<S, N2B, INV, SC, WHOLE, W == 3>(a, blockIdx.x);
 }
-template <int S, bool N2B, bool INV, int SC, bool WHOLE>
-__global__ __launch_bounds__(THREADS, SC == 2 && !WHOLE ? 2 : BA_CONTIG_WAVES) void ntt_pass_contig_group(const PassArgs *jobs, const u32 *wg_start,
-                                                                 u32 njobs) {
+template <int S, bool N2B, bool INV, int SC, bool WHOLE, int W>
+__global__ __launch_bounds__(THREADS, SC == 2 && !WHOLE ? 2 : W) void ntt_pass_contig_group(const PassArgs *jobs,
+                                                                                      const u32 *wg_start,
+                                                                                      u32 njobs) {
   const u32 j = group_of(wg_start, njobs, blockIdx.x);
-  contig_body<S, N2B, INV, SC, WHOLE>(jobs[j], blockIdx.x - wg_start[j]);
+  contig_body<S, N2B, INV, SC, WHOLE, W == 3>(jobs[j], blockIdx.x - wg_start[j]);
 }
 template <int K, bool N2B, bool INV>
 __global__ __launch_bounds__(THREADS) void ntt_tiny(PassArgs a) {

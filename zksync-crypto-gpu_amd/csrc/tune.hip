@@ -1,5 +1,6 @@
-// tune.hip -- the autotuner: per size class (log_n, input order) it times every pass split and
-// then the columns-per-workgroup of each pass, and keeps the fastest on the context.
+// tune.hip -- the autotuner: per size class (log_n, input order) it times every pass split, then
+// the columns-per-workgroup of each pass, then the waves per SIMD the contiguous pass runs at,
+// and keeps the fastest on the context.
 //
 // No reference counterpart: boojum-cuda's kernel shapes per log_n are fixed tables
 // (crates/boojum-cuda/src/ntt.rs:57-176).  The candidate splits are those the pass kernels
@@ -129,6 +130,19 @@ int tune_class(ba_context *ctx, uint32_t log_n, bool n2b, const uint64_t *in, ui
       if (ms < best_ms * 0.99f) best_ms = ms, best = e;
     }
   }
+  // then the waves per SIMD of a 9..12-bit contiguous pass (3 with the next-tile prefetch, or 4)
+  bool has_hi_contig = false;
+  for (int p = 0; p < best.plan.num_passes; p++) has_hi_contig |= best.plan.contig[p] && best.plan.bits[p] >= 9;
+  if (has_hi_contig) {
+    for (uint32_t w : {3u, 4u}) {
+      BaTuned e = best;
+      e.contig_waves = w;
+      float ms = 0;
+      const int rc = time_entry(ctx, e, log_n, n2b, in, out, cols, timer, ms);
+      if (rc) return rc;
+      if (ms < best_ms * 0.99f) best_ms = ms, best = e;
+    }
+  }
   ctx->tuned[n2b][log_n] = best;
   return BA_SUCCESS;
 }
@@ -181,6 +195,7 @@ extern "C" int ba_tuning_get(ba_context_t ctx, uint32_t log_n, int bitrev_inputs
       plan->bits[low] = p.bits[i];
       plan->reps[low] = t.valid ? t.reps[i] : 0;
     }
+    plan->contig_waves = t.valid ? t.contig_waves : 0;
     return BA_SUCCESS;
   });
 }
@@ -202,6 +217,8 @@ extern "C" int ba_tuning_set(ba_context_t ctx, uint32_t log_n, int bitrev_inputs
       const int exec = n2b ? (int)plan->num_passes - 1 - (int)i : (int)i;
       e.reps[exec] = plan->reps[i];
     }
+    if (plan->contig_waves != 0 && plan->contig_waves != 3 && plan->contig_waves != 4) return BA_ERROR_INVALID_VALUE;
+    e.contig_waves = plan->contig_waves;
     e.valid = true;
     ctx->tuned[n2b][log_n] = e;
     return BA_SUCCESS;

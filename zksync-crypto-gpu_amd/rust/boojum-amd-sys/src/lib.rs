@@ -54,6 +54,7 @@ pub struct ba_tuned_plan {
     pub num_passes: u32,
     pub bits: [u32; 8],
     pub reps: [u32; 8],
+    pub contig_waves: u32,
 }
 
 #[repr(C)]
