@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a 2^27-element copy kernel (canonicalise, 8 B read + 8 B written "
                          "per element) that scales the FETCH_SIZE/WRITE_SIZE counters")
+    ap.add_argument("--only-classes", default=None, metavar="LO:HI",
+                    help="profiling aid: only forward natural-order batch NTTs of 2^26 elements per size "
+                         "class log_n in LO..HI (2^(26-log_n) columns), --steps launches each")
     ap.add_argument("--serial-cosets", action="store_true",
                     help="run every LDE's coset NTTs in order on the caller's stream (sets "
                          "BOOJUM_AMD_LDE_SERIAL=1), so that a profiler's per-kernel durations are each "
@@ -532,6 +535,39 @@ def run_serve(ba, ctx, n_requests=240, rate_hz=40.0, seed=5, budget_frac=0.6, sl
             "mean_batch_jobs": round(st["mean_batch_jobs"], 2), "failed": st["failed"]}
 
 
+def run_classes(ba, ctx, args):
+    """Per size class log_n: a forward natural -> bit-reversed batch NTT over 2^26 elements
+    (2^(26 - log_n) columns, built-in plan), args.steps launches after one warm-up; prints one
+    JSON line with ms and Melem/s per class (a profiling aid for per-class counters)."""
+    lo, hi = (int(x) for x in args.only_classes.split(":"))
+    stream = ba.CudaStream()
+    total = 1 << 26
+    d_in, d_out = ba.DeviceAllocation(total), ba.DeviceAllocation(total)
+    ba.runtime.fill_uniform(d_in, total, 1, 11)
+    out = {}
+    for log_n in range(lo, hi + 1):
+        n, cols = 1 << log_n, total >> log_n
+        call = lambda: ba.ntt.batch_ntt_ctx(ctx, d_in.ptr, d_out.ptr, log_n, cols, n, n, False, False,  # noqa: E731
+                                            stream=stream)
+        call()
+        stream.synchronize()
+        e0, e1 = ba.CudaEvent(), ba.CudaEvent()
+        e0.record(stream)
+        for _ in range(args.steps):
+            call()
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / args.steps
+        out[str(log_n)] = {"columns": cols, "ms": round(ms, 4), "melem_s": round(total / ms / 1e3, 1),
+                           "plan": ba.tune.get_plan(ctx, log_n, False)[0]}
+    d_in.free()
+    d_out.free()
+    stream.destroy()
+    print(json.dumps({"metric": "batch NTT per size class (profiling aid)", "unit": "ms", "steps": args.steps,
+                      "config": {"workload": "forward natural-order batch NTT, 2^26 elements per class"},
+                      "classes": out}), flush=True)
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -548,6 +584,9 @@ def main():
     assert ba.device_count() >= 1, "no GPU visible"
     ba.set_device(0)
     ctx = ba.Context.create(12, 12)
+    if args.only_classes:
+        run_classes(ba, ctx, args)
+        return
     log_n, cols, lb = args.log_n, args.cols, args.log_blowup
     n, B = 1 << log_n, 1 << lb
     tuning = None
@@ -728,9 +767,9 @@ def main():
                      "traffic": traffic, "avg_launch_ms": round(dom_avg_ms, 4),
                      "alg_bytes_per_launch": dom_bytes,
                      # which ceiling the kernel is nearer: its VALU rate against the measured VALU
-                     # ceiling, or its HBM rate against the HBM peak
-                     "limited_by": ("valu" if valu is not None and valu["frac"] > achieved / HBM_PEAK_GBS
-                                    else "hbm"),
+                     # ceiling, or its HBM rate against the HBM peak (None without the VALU counters)
+                     "limited_by": (None if valu is None else
+                                    "valu" if valu["frac"] > achieved / HBM_PEAK_GBS else "hbm"),
                      # useful work: radix-2 butterflies (n/2 per stage per column) per second
                      "butterflies_per_s": round(dom_bfly / (dom_avg_ms / 1e3), 1),
                      "element_stages_per_s": round(2 * dom_bfly / (dom_avg_ms / 1e3), 1),
