@@ -21,6 +21,20 @@ constexpr u64 EPS = 0xFFFFFFFFull;  // 2^64 mod p
 
 __host__ __device__ __forceinline__ u64 canon(u64 x) { return x >= P ? x - P : x; }
 
+// Device memory read and written through the global address space.  Through a generic pointer
+// (one loaded from memory, or derived from one) the compiler emits flat_* instructions, which
+// count against lgkmcnt as well as vmcnt: every wait for an LDS access then also waits for the
+// flat loads and stores still in flight, serialising a tile's stores behind its LDS reads.
+// global_* instructions count against vmcnt only.
+template <typename T>
+__device__ __forceinline__ T gload(const T *p) {
+  return *(const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T *p, T v) {
+  *(__attribute__((address_space(1))) T *)p = v;
+}
+
 // Single gfx950 instructions, one asm statement each, with carries and lane masks in SGPR pairs:
 // the compiler schedules them like any other instruction (interleaving independent work) and
 // inserts the wait states a VALU-written SGPR needs before a VALU or SALU instruction reads it.

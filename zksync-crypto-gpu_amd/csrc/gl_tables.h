@@ -30,9 +30,9 @@ struct Tables {
 // base^e for e < 2^30 (the exponent is reduced mod 2^30 first)
 __device__ __forceinline__ u64 table_pow(const PowTable &t, u32 e) {
   e &= (1u << OMEGA_LOG_ORDER) - 1;
-  const u64 c = t.coarse[e >> t.fine_log];
+  const u64 c = gload(t.coarse + (e >> t.fine_log));
   const u32 f = e & t.fine_mask;
-  return f ? mul(t.fine[f], c) : c;
+  return f ? mul(gload(t.fine + f), c) : c;
 }
 
 // omega_{2^30}^(+-e)

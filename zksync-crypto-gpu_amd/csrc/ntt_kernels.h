@@ -276,10 +276,10 @@ struct PassArgs {
 };
 
 __device__ __forceinline__ const u64 *col_in(const PassArgs &a, u64 col) {
-  return a.in_cols ? a.in_cols[col] : a.in + col * a.stride_in;
+  return a.in_cols ? gload(a.in_cols + col) : a.in + col * a.stride_in;
 }
 __device__ __forceinline__ u64 *col_out(const PassArgs &a, u64 col) {
-  return a.out_cols ? a.out_cols[col] : a.out + col * a.stride_out;
+  return a.out_cols ? gload(a.out_cols + col) : a.out + col * a.stride_out;
 }
 
 // F(j0) = s^j0 (forward) or s^-j0 (inverse), or 1 without coset
@@ -416,12 +416,12 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
         const u32 e = t + THREADS * k;
         const u32 cl = e >> S, row = e & ((1u << S) - 1);
         const u64 *chunk = in + ((u64)bitrev32(l0 + cl, P0) << S);
-        nxt[k] = chunk[bitrev32(row, S)];
+        nxt[k] = gload(chunk + bitrev32(row, S));
       });
     } else {
       static_for<VALS>([&](auto m_) {
         constexpr int m = decltype(m_)::value;
-        nxt[m] = *at(in + ((u64)m << (WB0 + P0)), off0);
+        nxt[m] = gload(at(in + ((u64)m << (WB0 + P0)), off0));
       });
     }
   };
@@ -474,7 +474,7 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
     static_for<VALS>([&](auto m_) {
       constexpr int m = decltype(m_)::value;
       u64 *bm = out + ((u64)m << (WBL + P0));
-      *at(bm, offL) = v[m];
+      gstore(at(bm, offL), v[m]);
     });
   }
 }
@@ -580,7 +580,7 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
       const u64 *in = col_in(a, col) + tile_in_col;
       static_for<VALS>([&](auto k_) {
         constexpr int k = decltype(k_)::value;
-        nxt[k] = *at(in + THREADS * k, t * 8u);
+        nxt[k] = gload(at(in + THREADS * k, t * 8u));
       });
     } else {
       const u64 g0 = tile << LOG_C;
@@ -588,7 +588,7 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
         constexpr int k = decltype(k_)::value;
         const u32 e = t + THREADS * k;
         const u64 g = g0 + (e >> S);
-        nxt[k] = g < total_chunks ? col_in(a, g)[e & ((1u << S) - 1)] : 0;
+        nxt[k] = g < total_chunks ? gload(col_in(a, g) + (e & ((1u << S) - 1))) : 0;
       });
     }
   };
@@ -606,10 +606,11 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
     static_for<VALS>([&](auto k_) {
       constexpr int k = decltype(k_)::value;
       const u32 e = t + THREADS * k;
-      u32 slot = e;
-      // a bit-reversed input (single pass) is a permutation inside each column
-      if (N2B && a.in_bitrev) slot = ((e >> S) << S) + bitrev32(e & ((1u << S) - 1), L);
-      lds[lds_pad(slot)] = canon(nxt[k]);
+      // lds_pad(t + 256 k) = lds_pad(t) + 272 k; a bit-reversed input (single pass) is a
+      // permutation inside each column
+      u32 slot = lds_pad(t) + (THREADS + THREADS / 16) * k;
+      if (N2B && a.in_bitrev) slot = lds_pad(((e >> S) << S) + bitrev32(e & ((1u << S) - 1), L));
+      lds[slot] = canon(nxt[k]);
     });
     if (PREFETCH && i + 1 < reps) fetch(tile + (1ull << a.log_tile_stride));
     __syncthreads();
@@ -658,14 +659,14 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
       u64 *out = col_out(a, col) + tile_in_col;
       static_for<VALS>([&](auto k_) {
         constexpr int k = decltype(k_)::value;
-        *at(out + THREADS * k, t * 8u) = lds[lds_pad(t + THREADS * k)];
+        gstore(at(out + THREADS * k, t * 8u), lds[lds_pad(t) + (THREADS + THREADS / 16) * k]);
       });
     } else {
       static_for<VALS>([&](auto k_) {
         constexpr int k = decltype(k_)::value;
         const u32 e = t + THREADS * k;
         const u64 g = g0 + (e >> S);
-        if (g < total_chunks) col_out(a, g)[e & ((1u << S) - 1)] = lds[lds_pad(e)];
+        if (g < total_chunks) gstore(col_out(a, g) + (e & ((1u << S) - 1)), lds[lds_pad(e)]);
       });
     }
     __syncthreads();
@@ -689,7 +690,7 @@ __device__ __forceinline__ void tiny_body(const PassArgs &a, const u32 bid) {
   u64 v[VALS];
   static_for<(1 << K)>([&](auto i_) {
     constexpr int i = decltype(i_)::value;
-    u64 x = canon(in[a.in_bitrev ? cbitrev(i, K) : i]);
+    u64 x = canon(gload(in + (a.in_bitrev ? cbitrev(i, K) : i)));
     if (a.scale == SCALE_PRE) x = mul(x, tiny_factor(a, N2B ? i : cbitrev(i, K), false));
     v[i] = x;
   });
@@ -701,7 +702,7 @@ __device__ __forceinline__ void tiny_body(const PassArgs &a, const u32 bid) {
     constexpr int i = decltype(i_)::value;
     u64 x = v[i];
     if (a.scale == SCALE_POST) x = mul(x, tiny_factor(a, N2B ? cbitrev(i, K) : i, true));
-    out[i] = x;
+    gstore(out + i, x);
   });
 }
 
