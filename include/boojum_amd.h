@@ -271,7 +271,7 @@ int ba_batch_plan_destroy(ba_batch_plan_t plan);
  * choice), and the waves per SIMD of a 9..12-bit contiguous pass (contig_waves: 3 = the variant
  * that loads the next tile while transforming, 4 = the 128-register variant without that
  * prefetch, 0 = built-in choice: 3 below 2048 workgroups, else 4).  Valid splits: log_n <= 12
- * one pass of log_n bits; log_n > 12 a contiguous pass of 9..12 bits and strided passes of 4..8
+ * one pass of log_n bits; log_n > 12 a contiguous pass of 9..12 bits and strided passes of 4..10
  * bits (at most 8 passes).
  *
  * ba_autotune times every valid split, then each pass's reps, then contig_waves, for log_n in [log_n_min,

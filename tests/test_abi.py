@@ -87,7 +87,7 @@ def test_struct_layouts_match_header(tmp_path):
 
 def test_tuning_candidates_host_enumeration():
     """The library's candidate splits (host call, no device) equal the Python mirror's, and every
-    one covers log_n with a 9..12-bit contiguous pass under 4..8-bit strided passes."""
+    one covers log_n with a 9..12-bit contiguous pass under 4..10-bit strided passes."""
     from boojum_amd import tune
 
     for log_n in range(1, 29):
@@ -96,10 +96,11 @@ def test_tuning_candidates_host_enumeration():
         for bits in lib_c:
             assert sum(bits) == log_n
             if log_n > 12:
-                assert 9 <= bits[0] <= 12 and all(4 <= b <= 8 for b in bits[1:]) and len(bits) <= 8
+                assert 9 <= bits[0] <= 12 and all(4 <= b <= 10 for b in bits[1:]) and len(bits) <= 8
             else:
                 assert bits == [log_n]
     assert [12, 8] in tune.library_candidates(20)  # the built-in 2^20 plan is among them
+    assert [12, 10] in tune.library_candidates(22)  # two passes with a 10-bit strided pass
 
 
 def test_rust_sys_crate_declares_header():

@@ -242,3 +242,58 @@ def test_two_live_contexts(ba):
         ctx2.destroy()
         for t in (a, b, c):
             t.free()
+
+
+# Batches of at most 2^20 elements (log_n 4..18) run the latency-shaped passes (4 elements per
+# thread, 1024-element tiles); larger batches the throughput passes.  The reference has no such
+# split (ntt.rs:201-241 picks kernels by log_n alone); both shapes must give its results.
+LAT_MAX_LOG_ELEMS = 20
+
+
+def test_latency_passes_coset_single_column(ba):
+    correctness(ba, list(range(4, 19)), False, 2, 1, 1, seed=7)
+
+
+def test_latency_passes_inverse_coset_two_columns(ba):
+    correctness(ba, list(range(4, 18)), True, 2, 3, 2, seed=8)
+
+
+def _kernel_names(ba, fn):
+    from boojum_amd.profile import LaunchTrace
+    with LaunchTrace(16) as tr:
+        fn()
+    ba.synchronize()
+    names = list(tr.names)
+    tr.destroy()
+    return names
+
+
+@pytest.mark.parametrize("log_n,cols", [(16, 1), (12, 7), (18, 1), (18, 4), (18, 5), (16, 17), (19, 1)])
+def test_latency_path_choice_and_agreement(ba, log_n, cols):
+    """The path is chosen by batch size; at the boundary both paths give identical outputs
+    (one call over all columns against per-column calls, which run the latency shape)."""
+    n = 1 << log_n
+    a = ba.DeviceAllocation(n * cols)
+    b = ba.DeviceAllocation(n * cols)
+    c = ba.DeviceAllocation(n * cols)
+    try:
+        ba.runtime.fill_uniform(a, n, cols, 11)
+        for bitrev, inverse, le, ci in ((False, False, 0, 0), (True, True, 1, 1), (False, True, 2, 2),
+                                        (True, False, 3, 5)):
+            log(f"latency path log_n={log_n} cols={cols} bitrev={bitrev} inverse={inverse}")
+            names = _kernel_names(ba, lambda: ba.ntt.batch_ntt_internal(a.ptr, b.ptr, log_n, cols, n, n, bitrev,
+                                                                       inverse, le, ci))
+            lat = (cols << log_n) <= (1 << LAT_MAX_LOG_ELEMS) and 4 <= log_n <= 18
+            assert all(("ntt_pass_lat" in s) == lat for s in names), names
+            for k in range(cols):
+                ba.ntt.batch_ntt_internal(a.ptr + 8 * k * n, c.ptr + 8 * k * n, log_n, 1, n, n, bitrev, inverse,
+                                          le, ci)
+            ba.synchronize()
+            assert np.array_equal(b.to_host(), c.to_host())
+        # one column against the oracle
+        x = a.to_host(n)
+        want = gl_oracle.batch_ntt(x, log_n, 1, n, n, True, False, 3, 5, nthreads=8)
+        assert np.array_equal(b.to_host(n), want)
+    finally:
+        for t in (a, b, c):
+            t.free()

@@ -7,6 +7,8 @@ import pytest
 import gl_oracle
 from gpu_util import log, raw_u64
 
+LAT_MAX_LOG_ELEMS = 20  # batch NTTs of at most 2^20 elements run the latency-shaped passes
+
 pytestmark = pytest.mark.gpu
 
 
@@ -48,8 +50,9 @@ def test_default_plans(ba):
 def test_every_split_bit_exact(ba, log_n):
     """Each split the autotuner can choose, with reps cycling through 0/1/2/4/8 and the
     contiguous pass's waves per SIMD through built-in/3/4, in both input orders, forward with a
-    coset and inverse, against the oracle."""
-    cols = 3
+    coset and inverse, against the oracle.  Batches are larger than 2^20 elements: smaller ones
+    run the latency-shaped passes, which no plan selects."""
+    cols = max(3, (1 << (LAT_MAX_LOG_ELEMS + 1) >> log_n) + 1)
     n = 1 << log_n
     rng = np.random.default_rng(log_n)
     x = raw_u64(rng, cols * n)
@@ -72,13 +75,47 @@ def test_every_split_bit_exact(ba, log_n):
         ba.tune.set_plan(ba._test_ctx, log_n, br, None)
 
 
+@pytest.mark.parametrize("log_n,bits", [(19, [10, 9]), (19, [9, 10]), (22, [12, 10]), (21, [11, 10]),
+                                         (22, [12, 5, 5])])
+def test_wide_strided_passes_bit_exact(ba, log_n, bits):
+    """The 9- and 10-bit strided passes (runs of 8 and 4 elements per row), alone above the
+    contiguous pass or under another, in every mode including the LDE's gathered coset pass,
+    against the oracle on sampled columns."""
+    cols = 2
+    n = 1 << log_n
+    rng = np.random.default_rng(log_n + bits[-1])
+    x = raw_u64(rng, cols * n)
+    modes = [(False, False, 1, 1), (True, True, 0, 0), (False, True, 2, 3), (True, False, 3, 5)]
+    for m in modes:
+        log(f"wide strided log_n={log_n} bits={bits} mode={m}")
+        ba.tune.set_plan(ba._test_ctx, log_n, m[0], bits, [0] * len(bits))
+        got = run_ntt(ba, x, log_n, cols, *m)
+        want = gl_oracle.batch_ntt(x, log_n, cols, n, n, m[0], m[1], m[2], m[3], nthreads=8)
+        assert np.array_equal(got, want), (bits, m)
+    # the LDE's coset NTTs gather the bit-reversed monomials in their first (top strided) pass
+    ev = raw_u64(rng, n)
+    d_in = ba.DeviceAllocation.from_host(ev)
+    d_out = ba.DeviceAllocation(2 * n)
+    try:
+        ba.lde.lde_batch(ba._test_ctx, d_in, log_n, 1, 1, d_out)
+        ba.synchronize()
+        got = d_out.to_host()
+        want = gl_oracle.lde(ev, log_n, 1, 1, nthreads=8)
+        assert np.array_equal(got, want), bits
+    finally:
+        d_in.free()
+        d_out.free()
+    for br in (False, True):
+        ba.tune.set_plan(ba._test_ctx, log_n, br, None)
+
+
 @pytest.mark.parametrize("log_n", [13, 16])
 def test_scaled_contiguous_pass_columns_per_workgroup(ba, log_n):
     """The two modes whose per-element coset factors are fused into the contiguous pass
     (forward b2n: before its first round; inverse n2b: after its last round) with 1, 2, 4 and 8
     columns per workgroup: each workgroup forms the factors anew for every column it repeats
     over, and every column matches the oracle."""
-    cols = 8
+    cols = max(8, (1 << (LAT_MAX_LOG_ELEMS + 1) >> log_n) + 1)
     n = 1 << log_n
     rng = np.random.default_rng(100 + log_n)
     x = raw_u64(rng, cols * n)
@@ -98,7 +135,7 @@ def test_set_plan_rejects_invalid(ba):
     bad = [
         (20, [12, 7]),        # does not cover 20 bits
         (20, [8, 8, 4]),      # contiguous pass below 9 bits for log_n > 12
-        (20, [11, 9]),        # strided pass above 8 bits
+        (20, [9, 11]),        # strided pass above 10 bits
         (20, [12, 3, 5]),     # strided pass below 4 bits
         (10, [6, 4]),         # log_n <= 12 is one pass
         (13, [13]),           # contiguous pass above 12 bits
@@ -129,7 +166,8 @@ def test_autotune_then_bit_exact(ba):
             assert all(r in (0, 1, 2, 4, 8) for r in reps)
     rng = np.random.default_rng(3)
     for log_n in (10, 13, 17):
-        cols, n = 2, 1 << log_n
+        n = 1 << log_n
+        cols = (1 << (LAT_MAX_LOG_ELEMS + 1) >> log_n) + 1  # past the latency-shaped passes
         x = raw_u64(rng, cols * n)
         for m in [(False, False, 1, 0), (True, True, 0, 0)]:
             want = gl_oracle.batch_ntt(x, log_n, cols, n, n, m[0], m[1], m[2], m[3], nthreads=8)

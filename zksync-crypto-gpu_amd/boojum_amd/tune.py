@@ -65,14 +65,14 @@ def candidate_splits(log_n):
             if rest == 0:
                 out.append(list(cur))
             return
-        for b in range(4, min(8, rest) + 1):
+        for b in range(4, min(10, rest) + 1):
             comps(rest - b, parts - 1, cur + [b])
 
     for sc in range(9, 13):
         if log_n < sc + 4:
             continue
         rest = log_n - sc
-        min_parts = (rest + 7) // 8
+        min_parts = (rest + 9) // 10
         for parts in range(min_parts, min(min_parts + 1, 7) + 1):
             before = len(out)
             comps(rest, parts, [])

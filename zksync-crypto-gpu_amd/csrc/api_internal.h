@@ -95,10 +95,17 @@ struct BaPassLaunch {
 };
 
 // Appends the pass launches of one batch NTT (arguments validated) to `launches`.
+// allow_latency: a batch of at most 2^BA_LAT_MAX_LOG_ELEMS elements runs the latency-shaped
+// passes (single-batch kernels only: callers that group launches leave it false).
 int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint32_t log_n,
                  uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs,
                  bool inverse, uint32_t log_ext, uint32_t coset_idx, bool in_bitrev_gather,
-                 std::vector<BaPassLaunch> &launches);
+                 std::vector<BaPassLaunch> &launches, bool allow_latency = false);
+// Per-call time over log_n 10..18 and batches of 2^14..2^22 elements (MI355X): up to 2^20
+// elements the latency passes take 1.0-3.1x less time per call, above it the throughput passes win.
+constexpr uint32_t BA_LAT_MAX_LOG_ELEMS = 20;
+// The batch NTT of num_cols columns of 2^log_n elements runs the latency-shaped passes.
+bool ba_use_latency_passes(uint32_t log_n, uint32_t num_cols);
 // Launches them one after another on `stream` (single-batch kernels).
 int ba_run_launches(const std::vector<BaPassLaunch> &launches, hipStream_t stream);
 uint32_t ba_pick_reps(uint64_t tiles, uint32_t cols);

@@ -8,11 +8,27 @@ namespace gl_host {
 typedef unsigned __int128 u128;
 constexpr uint64_t P = 0xFFFFFFFF00000001ull;
 
-inline uint64_t mul(uint64_t a, uint64_t b) { return (uint64_t)(((u128)a * b) % P); }
+constexpr uint64_t EPS = 0xFFFFFFFFull;  // 2^64 mod p
+
+// Canonical residue of a 128-bit value x = lo + hl 2^64 + hh 2^96 (hl, hh the 32-bit halves of
+// the high word): 2^64 = EPS and 2^96 = -1 mod p, so x = lo - hh + hl EPS.  No division: called
+// per launch (scale tables), where a 128-bit % costs tens of nanoseconds.
+inline uint64_t reduce128(u128 x) {
+  const uint64_t lo = (uint64_t)x, hi = (uint64_t)(x >> 64);
+  const uint64_t hl = hi & EPS, hh = hi >> 32;
+  uint64_t t = lo - hh;
+  if (lo < hh) t -= EPS;  // borrowed 2^64 = EPS: t = lo - hh + 2^64 >= 2^64 - 2^32 > EPS
+  const uint64_t u = hl * EPS;
+  uint64_t r = t + u;
+  if (r < u) r += EPS;  // carried 2^64: r + EPS < 2^64 since t + u < 2^64 + 2^64 - 2^33
+  return r >= P ? r - P : r;
+}
+
+inline uint64_t mul(uint64_t a, uint64_t b) { return reduce128((u128)a * b); }
 
 inline uint64_t pow(uint64_t a, uint64_t e) {
   uint64_t r = 1;
-  a %= P;
+  a = a >= P ? a - P : a;
   while (e) {
     if (e & 1) r = mul(r, a);
     a = mul(a, a);

@@ -6,11 +6,13 @@
 // passes of radix-16 register blocks (see ntt_kernels.h).
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "api_internal.h"
 #include "gl_host.h"
 #include "ntt_inst.h"
+#include "ntt_lat.h"
 
 using gl::ntt::PassArgs;
 
@@ -22,11 +24,11 @@ bool ba_plan_from_bits(uint32_t log_n, bool n2b, const uint32_t *bits, int num_p
   } else if (log_n <= T) {  // one contiguous pass
     if (num_passes != 1 || bits[0] != log_n) return false;
   } else {
-    // contiguous pass over 9..12 low bits (whole 4096-element tiles), strided passes of 4..8
+    // contiguous pass over 9..12 low bits (whole 4096-element tiles), strided passes of 4..10
     if (num_passes < 2 || bits[0] < 9 || bits[0] > T) return false;
     uint32_t sum = bits[0];
     for (int i = 1; i < num_passes; i++) {
-      if (bits[i] < 4 || bits[i] > 8) return false;
+      if (bits[i] < 4 || bits[i] > 10) return false;
       sum += bits[i];
     }
     if (sum != log_n) return false;
@@ -135,10 +137,39 @@ uint32_t ba_pick_reps(uint64_t tiles, uint32_t cols) {
   return reps;
 }
 
+// Latency-shaped plan (ntt_lat.h): the bits of each pass from the low bits up (a contiguous
+// pass of <= 10 bits, then a strided pass of <= 8), or 0 when the size has none.
+static int lat_bits(uint32_t log_n, uint32_t *bits) {
+  if (log_n < 4 || log_n > 18) return 0;
+  if (log_n <= 10) {
+    bits[0] = log_n;
+    return 1;
+  }
+  bits[1] = std::min<uint32_t>(8, log_n / 2);
+  bits[0] = log_n - bits[1];
+  return 2;
+}
+
+// BOOJUM_AMD_LAT_MAX_LOG_ELEMS overrides the batch-size bound (read once; 0 turns the latency
+// passes off)
+static uint32_t lat_max_log_elems() {
+  static const uint32_t v = [] {
+    const char *e = std::getenv("BOOJUM_AMD_LAT_MAX_LOG_ELEMS");
+    return e && *e ? (uint32_t)std::strtoul(e, nullptr, 10) : BA_LAT_MAX_LOG_ELEMS;
+  }();
+  return v;
+}
+
+bool ba_use_latency_passes(uint32_t log_n, uint32_t num_cols) {
+  uint32_t bits[2];
+  const uint32_t m = lat_max_log_elems();
+  return m > 0 && lat_bits(log_n, bits) > 0 && ((uint64_t)num_cols << log_n) <= (1ull << std::min(m, 40u));
+}
+
 int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint32_t log_n,
                  uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs,
                  bool inverse, uint32_t log_ext, uint32_t coset_idx, bool in_bitrev_gather,
-                 std::vector<BaPassLaunch> &launches) {
+                 std::vector<BaPassLaunch> &launches, bool allow_latency) {
   if (num_cols == 0) return BA_SUCCESS;
   const bool n2b = !bitrev_inputs;
   PassArgs base{};
@@ -179,8 +210,50 @@ int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint3
     return BA_SUCCESS;
   }
 
-  const BaTuned &tuning = ctx->tuned[n2b][log_n];
-  const NttPlan plan = tuning.valid ? tuning.plan : ba_plan_ntt(log_n, n2b);
+  if (allow_latency && !in_bitrev_gather && ba_use_latency_passes(log_n, num_cols)) {
+    uint32_t bits[2];
+    const int np = lat_bits(log_n, bits);
+    for (int p = 0; p < np; p++) {
+      const int src = n2b ? np - 1 - p : p;  // n2b: top bits first
+      const uint32_t S = bits[src], lo = src == 0 ? 0 : bits[0];
+      const bool strided = src > 0, top = lo + S == log_n;
+      BaPassLaunch l;
+      PassArgs &a = l.args;
+      a = base;
+      a.in = p == 0 ? in : out;
+      a.stride_in = p == 0 ? stride_in : stride_out;
+      a.out = out;
+      a.stride_out = stride_out;
+      a.num_cols = num_cols;
+      a.lo_bit = lo;
+      a.reps = 1;
+      a.scale = (pre && p == 0) ? SCALE_PRE : ((inverse && p == np - 1) ? SCALE_POST : SCALE_NONE);
+      const bool scale = a.scale != SCALE_NONE;
+      const uint64_t chunks = (uint64_t)num_cols << (log_n - S);
+      l.grid = strided ? ((uint64_t)num_cols << (log_n - LAT_TL))
+                       : (chunks + (1u << (LAT_TL - S)) - 1) >> (LAT_TL - S);
+      a.tile_count = (uint32_t)l.grid;
+      l.kernels = ba_pick_lat(S, strided, n2b, inverse, top, scale);
+      l.name = ba_lat_kernel_name(S, strided, n2b, inverse, top, scale);
+      l.pass = p;
+      l.col0 = 0;
+      if (!l.kernels.single) return BA_ERROR_INVALID_VALUE;
+      launches.push_back(l);
+    }
+    return BA_SUCCESS;
+  }
+
+  // The gathered first pass of an LDE's coset NTT has no 9- or 10-bit variant (the gathered and
+  // scaled wide pass measured 11 % slower than the built-in split at 2^21): a tuned plan with such
+  // a pass gives way to the built-in plan there.
+  bool tuned = ctx->tuned[n2b][log_n].valid;
+  if (tuned && in_bitrev_gather) {
+    const NttPlan &tp = ctx->tuned[n2b][log_n].plan;
+    for (int p = 0; p < tp.num_passes; p++)
+      if (!tp.contig[p] && tp.bits[p] > 8) tuned = false;
+  }
+  const BaTuned &tuning = tuned ? ctx->tuned[n2b][log_n] : BaTuned{};
+  const NttPlan plan = tuned ? tuning.plan : ba_plan_ntt(log_n, n2b);
   for (int p = 0; p < plan.num_passes; p++) {
     const uint32_t S = plan.bits[p];
     const bool contig = plan.contig[p];
@@ -205,8 +278,8 @@ int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint3
       a.lo_bit = plan.lo_bit[p];
       a.tile_count = (uint32_t)tiles;
       a.log_tile_stride = log_stride;
-      const uint32_t tuned = tuning.valid ? tuning.reps[p] : 0;
-      a.reps = tuned ? std::min<uint32_t>(tuned, big ? nc : (uint32_t)tiles)
+      const uint32_t tuned_reps = tuning.valid ? tuning.reps[p] : 0;
+      a.reps = tuned_reps ? std::min<uint32_t>(tuned_reps, big ? nc : (uint32_t)tiles)
                      : ba_pick_reps(tiles, big ? nc : (uint32_t)tiles);
       if (a.reps == 0) a.reps = 1;
       a.scale = (pre && p == 0) ? SCALE_PRE
@@ -251,7 +324,7 @@ int ba_launch_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint
                   bool in_bitrev_gather) {
   std::vector<BaPassLaunch> launches;
   const int rc = ba_build_ntt(ctx, in, out, log_n, num_cols, stride_in, stride_out, bitrev_inputs,
-                              inverse, log_ext, coset_idx, in_bitrev_gather, launches);
+                              inverse, log_ext, coset_idx, in_bitrev_gather, launches, true);
   if (rc) return rc;
   return ba_run_launches(launches, stream);
 }

@@ -20,8 +20,14 @@ struct BaKernels {
 BaKernels ba_pick_contig_lo(uint32_t S, bool n2b, bool inverse, int scale, bool whole);
 BaKernels ba_pick_contig_hi(uint32_t S, bool n2b, bool inverse, int scale, bool whole);   // 4 waves/SIMD
 BaKernels ba_pick_contig_hi3(uint32_t S, bool n2b, bool inverse, int scale, bool whole);  // 3, prefetch
-// strided pass, 4 <= S <= 8; top: the pass holds the top bits; scale: fused (pre/post) scale;
+// strided pass, 4 <= S <= 10; top: the pass holds the top bits; scale: fused (pre/post) scale;
 // gather: the n2b top pass reads bit-reversed columns
 BaKernels ba_pick_strided(uint32_t S, bool n2b, bool inverse, bool top, bool scale, bool gather);
 // whole transform of log_n <= 3 per thread
 BaKernels ba_pick_tiny(uint32_t log_n, bool n2b, bool inverse);
+// latency-shaped passes (ntt_lat.h) for batches too small to fill the chip: contiguous over
+// 4..10 bits, strided over 5..8 bits; no grouped variant (batch plans use the passes above)
+constexpr int BA_LAT_MIN_CONTIG = 4, BA_LAT_MAX_CONTIG = 10;
+constexpr int BA_LAT_MIN_STRIDED = 5, BA_LAT_MAX_STRIDED = 8;
+BaKernels ba_pick_lat(uint32_t S, bool strided, bool n2b, bool inverse, bool top, bool scale);
+const char *ba_lat_kernel_name(uint32_t S, bool strided, bool n2b, bool inverse, bool top, bool scale);

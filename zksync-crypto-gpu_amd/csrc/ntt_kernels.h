@@ -14,8 +14,10 @@
 // index and LDS offset is a compile-time constant) with a 4096-element tile per 256-thread
 // workgroup, 16 elements per thread in registers, LDS transposes between radix-16 rounds:
 //   * contiguous pass: bits [0, S), each tile holds 4096/2^S contiguous chunks of 2^S;
-//   * strided pass:    bits [P0, P0 + S), S <= 8, each tile holds 2^S rows of 4096/2^S >= 16
-//                      consecutive elements (128 B+ runs), read and written in place.
+//   * strided pass:    bits [P0, P0 + S), S <= 10, each tile holds 2^S rows of 4096/2^S >= 4
+//                      consecutive elements (128 B+ runs up to S = 8, 64 and 32 B runs for the
+//                      9- and 10-bit passes that let 2^21 and 2^22 run in two passes), read
+//                      and written in place.
 // A workgroup repeats its tile over several columns (tiles that share twiddles), computing the
 // round twiddles and scale factors once and keeping them in registers.
 // n2b runs its strided passes (high bits first) then the contiguous pass; b2n runs the
@@ -367,7 +369,8 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
   static_assert(!GATHER || (N2B && TOP), "only the n2b top pass gathers");
   using Sh = Shape<S, N2B>;
   constexpr int R = Sh::ROUNDS;
-  constexpr int LOG_C = TILE_LOG - S;  // consecutive l values per tile (>= 16)
+  constexpr int LOG_C = TILE_LOG - S;  // consecutive l values per tile (>= 4)
+  static_assert(S >= 4 && S <= 10, "strided passes cover 4..10 bits");
   constexpr int C = 1 << LOG_C;
   constexpr int ROW = C + 1;           // padded LDS row (odd stride in u64)
   __shared__ u64 lds[(1 << S) * ROW];
@@ -721,13 +724,16 @@ __device__ __forceinline__ u32 group_of(const u32 *wg_start, u32 njobs, u32 b) {
 }
 
 // Strided passes of at most this many bits take the XCD-contiguous tile order (measured: the
-// 4-bit pass over 2^22-element columns 12-15 % faster; 6- and 8-bit passes neutral or slower).
+// 4-bit pass over 2^22-element columns 12-15 % faster; 6- and 8-bit passes neutral or slower),
+// and so do the 9- and 10-bit passes, whose rows are runs of 8 and 4 elements (64 and 32 B): the
+// neighbouring tiles that read the rest of each 128 B line then run on the same XCD, whose L2
+// holds it.
 #ifndef BA_XCD_REMAP_MAX_S
 #define BA_XCD_REMAP_MAX_S 4
 #endif
 template <int S, bool N2B, bool INV, bool TOP, bool SCALE, bool GATHER>
 __global__ __launch_bounds__(THREADS) void ntt_pass_strided(PassArgs a) {
-  if constexpr (S <= BA_XCD_REMAP_MAX_S) {
+  if constexpr (S <= BA_XCD_REMAP_MAX_S || S >= 9) {
     // Workgroups are dealt round-robin over the 8 XCDs, so neighbouring tiles (neighbouring runs
     // of the same 2^S rows, 2^(log_n - S) elements apart) would run at once on all eight and
     // crowd the same HBM channels.  XCD x instead takes the contiguous tile range starting at

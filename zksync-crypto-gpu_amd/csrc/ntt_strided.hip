@@ -9,11 +9,14 @@ static BaKernels pair() {
   return {ntt_pass_strided<S, N2B, INV, TOP, SCALE, GATHER>, ntt_pass_strided_group<S, N2B, INV, TOP, SCALE, GATHER>};
 }
 
+// the gathered top pass (LDE coset NTTs) exists for 4..8 bits only
 template <int S, bool N2B, bool INV>
 static BaKernels pick3(bool top, bool scale, bool gather) {
-  if (!top) return pair<S, N2B, INV, false, false, false>();
-  if constexpr (N2B) {
-    if (gather) return scale ? pair<S, N2B, INV, true, true, true>() : pair<S, N2B, INV, true, false, true>();
+  if (!top) return gather ? BaKernels{nullptr, nullptr} : pair<S, N2B, INV, false, false, false>();
+  if (gather) {
+    if constexpr (N2B && S <= 8)
+      return scale ? pair<S, N2B, INV, true, true, true>() : pair<S, N2B, INV, true, false, true>();
+    return {nullptr, nullptr};
   }
   return scale ? pair<S, N2B, INV, true, true, false>() : pair<S, N2B, INV, true, false, false>();
 }
@@ -31,6 +34,8 @@ BaKernels ba_pick_strided(uint32_t S, bool n2b, bool inverse, bool top, bool sca
     case 6: return pick<6>(n2b, inverse, top, scale, gather);
     case 7: return pick<7>(n2b, inverse, top, scale, gather);
     case 8: return pick<8>(n2b, inverse, top, scale, gather);
+    case 9: return pick<9>(n2b, inverse, top, scale, gather);
+    case 10: return pick<10>(n2b, inverse, top, scale, gather);
   }
   return {nullptr, nullptr};
 }

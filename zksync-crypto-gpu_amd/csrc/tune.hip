@@ -5,7 +5,7 @@
 // No reference counterpart: boojum-cuda's kernel shapes per log_n are fixed tables
 // (crates/boojum-cuda/src/ntt.rs:57-176).  The candidate splits are those the pass kernels
 // cover: one contiguous pass for log_n <= 12; otherwise a contiguous pass over 9..12 low bits
-// (whole 4096-element tiles) below strided passes of 4..8 bits each, from the fewest strided
+// (whole 4096-element tiles) below strided passes of 4..10 bits each, from the fewest strided
 // passes that cover the remaining bits to one more than that (an extra pass adds HBM traffic
 // but can pay for itself when the passes are bound by integer ALU work).
 #include <algorithm>
@@ -15,14 +15,14 @@
 
 namespace {
 
-// every composition of `rest` into `parts` pieces of 4..8 bits
+// every composition of `rest` into `parts` pieces of 4..10 bits
 void compositions(uint32_t rest, int parts, std::vector<uint32_t> &cur,
                   std::vector<std::vector<uint32_t>> &out) {
   if (parts == 0) {
     if (rest == 0) out.push_back(cur);
     return;
   }
-  for (uint32_t b = 4; b <= 8 && b <= rest; b++) {
+  for (uint32_t b = 4; b <= 10 && b <= rest; b++) {
     cur.push_back(b);
     compositions(rest - b, parts - 1, cur, out);
     cur.pop_back();
@@ -39,7 +39,7 @@ std::vector<std::vector<uint32_t>> candidate_splits(uint32_t log_n) {
   for (uint32_t sc = 9; sc <= (uint32_t)gl::ntt::TILE_LOG; sc++) {
     if (log_n < sc + 4) continue;
     const uint32_t rest = log_n - sc;
-    const int min_parts = (int)((rest + 7) / 8);
+    const int min_parts = (int)((rest + 9) / 10);
     for (int parts = min_parts; parts <= min_parts + 1 && parts <= 7; parts++) {
       std::vector<std::vector<uint32_t>> comps;
       std::vector<uint32_t> cur;
