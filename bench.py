@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--only-classes", default=None, metavar="LO:HI",
                     help="profiling aid: only forward natural-order batch NTTs of 2^26 elements per size "
                          "class log_n in LO..HI (2^(26-log_n) columns), --steps launches each")
+    ap.add_argument("--only-single", action="store_true",
+                    help="run only the single 2^16 NTT leg (one call per transform) and print its JSON line")
     ap.add_argument("--serial-cosets", action="store_true",
                     help="run every LDE's coset NTTs in order on the caller's stream (sets "
                          "BOOJUM_AMD_LDE_SERIAL=1), so that a profiler's per-kernel durations are each "
@@ -586,6 +588,11 @@ def main():
     ctx = ba.Context.create(12, 12)
     if args.only_classes:
         run_classes(ba, ctx, args)
+        return
+    if args.only_single:
+        stream = ba.CudaStream()
+        print(json.dumps({"metric": "single NTT per call", "single": run_single(ba, ctx, stream)}), flush=True)
+        stream.destroy()
         return
     log_n, cols, lb = args.log_n, args.cols, args.log_blowup
     n, B = 1 << log_n, 1 << lb
