@@ -796,7 +796,14 @@ def main():
         if not args.no_tune:
             t0 = time.perf_counter()
             ba.tune.autotune(uctx, 12, 22, elems_per_trial=1 << 26)
-            utune = {"log_n_range": [12, 22], "seconds": round(time.perf_counter() - t0, 2)}
+            utune = {"log_n_range": [12, 22], "seconds": round(time.perf_counter() - t0, 2),
+                     # per size class and input order: pass split (low bits first), contiguous-pass
+                     # waves per SIMD, tile (10: 1024-element latency-shaped tiles at every batch
+                     # size, 0: by batch size)
+                     "choices": {f"{ln}{'b' if br else 'n'}": [ba.tune.get_plan(uctx, ln, br)[0],
+                                                             ba.tune.get_contig_waves(uctx, ln, br),
+                                                             ba.tune.get_tile_log(uctx, ln, br)]
+                                 for ln in range(12, 23) for br in (False, True)}}
         line["uneven"] = run_uneven(ba, uctx, stream)
         line["uneven"]["tuning"] = utune
         uctx.destroy()

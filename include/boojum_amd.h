@@ -270,11 +270,16 @@ int ba_batch_plan_destroy(ba_batch_plan_t plan);
  * per pass in the same order, the columns each workgroup repeats over (reps, 1..8; 0 = built-in
  * choice), and the waves per SIMD of a 9..12-bit contiguous pass (contig_waves: 3 = the variant
  * that loads the next tile while transforming, 4 = the 128-register variant without that
- * prefetch, 0 = built-in choice: 3 below 2048 workgroups, else 4).  Valid splits: log_n <= 12
+ * prefetch, 0 = built-in choice: 3 below 2048 workgroups, else 4), and the tile shape of plain
+ * batch NTTs (tile_log: 10 = the latency-shaped passes, 1024-element tiles of 4 elements per
+ * thread in radix-4 rounds, for every batch size, log_n 4..18 only; 12 = the 4096-element tiles
+ * of the split above for every batch size; 0 = built-in choice: 1024-element tiles for batches of
+ * at most 2^20 elements, else 4096).  LDE coset NTTs always run the 4096-element tiles.  Valid splits: log_n <= 12
  * one pass of log_n bits; log_n > 12 a contiguous pass of 9..12 bits and strided passes of 4..10
  * bits (at most 8 passes).
  *
- * ba_autotune times every valid split, then each pass's reps, then contig_waves, for log_n in [log_n_min,
+ * ba_autotune times every valid split, then each pass's reps, then contig_waves, then (log_n 4..18)
+ * the 1024-element tiles against the best of those, for log_n in [log_n_min,
  * log_n_max] and both input orders, on batches of max(elems_per_trial, 2^log_n) elements of
  * scratch memory it allocates and frees, and keeps the fastest on `ctx`; every later transform
  * (NTT, LDE, batch plan) on `ctx` uses it.  Results are bit-identical for every plan.  Tuning
@@ -284,6 +289,7 @@ typedef struct {
   uint32_t bits[8];
   uint32_t reps[8];
   uint32_t contig_waves; /* 0, 3 or 4 */
+  uint32_t tile_log;     /* 0, 10 or 12 */
 } ba_tuned_plan;
 int ba_autotune(ba_context_t ctx, uint32_t log_n_min, uint32_t log_n_max, uint64_t elems_per_trial,
                 ba_stream_t stream);
@@ -303,8 +309,10 @@ int ba_tuning_candidates(uint32_t log_n, ba_tuned_plan *plans, uint32_t capacity
  * request's working buffer (2^log_n x num_cols elements) and outputs (2^(log_n+log_blowup) x
  * num_cols) -- while the oldest queued request waits for memory, later requests that fit may pass
  * it, at most max_bypass of them (default 32, ba_server_set_max_bypass; 0 = strict arrival
- * order) --, merges everything admitted since its last launch into one batch plan and enqueues
- * it on a free slot stream (max_inflight_batches slots), so batches overlap on the device.
+ * order) --, merges the requests admitted since its last launch into batch plans, one per output
+ * size class (below 2^26 elements, below 2^30, from 2^30 up; the oldest admitted request's class
+ * first), and enqueues each on a free slot stream (max_inflight_batches slots), so batches overlap
+ * on the device and a small request never waits for a large one merged with it.
  * Results are the same as ba_lde_batch on the same evaluations: column k, coset c of the
  * output is at outputs + k * out_col_stride + c * 2^log_n (bit-reversed within the coset, the
  * cosets in shivini's order).  Outputs stay resident until ba_server_release.  A request

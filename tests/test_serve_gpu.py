@@ -281,3 +281,34 @@ def test_bypass_admits_small_requests_past_a_waiting_head(ba):
             log(f"bypass {bypass}: small queue ms {[round(r['queue_ms'], 2) for r in smalls]}")
         finally:
             srv.destroy()
+
+
+def test_batches_hold_one_size_class(ba):
+    """Requests submitted together but of different output size classes (below 2^26 elements,
+    below 2^30, from 2^30 up) run in different batches, so the small ones do not wait for the
+    large one merged with them; results match the oracle on sampled columns."""
+    shapes = [(20, 64, 2)] + [(12, 8, 1)] * 5 + [(14, 4, 2)] * 2  # 2^28 output, then < 2^26 each
+    srv = ba.serve.Server(ba._test_ctx, 8 << 30, max_inflight_batches=2)
+    try:
+        tickets = [srv.submit(log_n, cols, lb, seed=9300 + i) for i, (log_n, cols, lb) in enumerate(shapes)]
+        results = [srv.wait(t) for t in tickets]
+        classes = {}
+        for (log_n, cols, lb), res in zip(shapes, results):
+            assert res["status"] == 0, res
+            e = cols << (log_n + lb)
+            classes.setdefault(0 if e < 1 << 26 else (1 if e < 1 << 30 else 2), set()).add(res["batch"])
+        log(f"batches by size class: {classes}")
+        assert len(classes) == 2 and not (classes[0] & classes[1])
+        for i in (0, 1, len(shapes) - 1):
+            log_n, cols, lb = shapes[i]
+            m = 1 << (log_n + lb)
+            k = cols - 1
+            got = srv.output(results[i], log_n, cols, lb).sub(k * m, m).to_host()
+            x = gl_oracle.fill_uniform(1 << log_n, column_seed(9300 + i, k))
+            assert np.array_equal(got, gl_oracle.lde(x, log_n, 1, lb, nthreads=8)), shapes[i]
+        for t in tickets:
+            srv.release(t)
+        st = srv.stats()
+        assert st["failed"] == 0 and st["in_use_bytes"] == 0
+    finally:
+        srv.destroy()

@@ -164,6 +164,7 @@ def test_autotune_then_bit_exact(ba):
             log(f"tuned log_n={log_n} bitrev={br}: bits={bits} reps={reps}")
             assert sum(bits) == log_n and bits in ba.tune.candidate_splits(log_n)
             assert all(r in (0, 1, 2, 4, 8) for r in reps)
+            assert ba.tune.get_tile_log(ba._test_ctx, log_n, br) in (0, 10)
     rng = np.random.default_rng(3)
     for log_n in (10, 13, 17):
         n = 1 << log_n
@@ -187,3 +188,92 @@ def test_autotune_rejects_bad_range(ba):
     for lo, hi in [(0, 5), (9, 8), (20, 29)]:
         with pytest.raises(ba.BoojumAmdError):
             ba.tune.autotune(ba._test_ctx, lo, hi, elems_per_trial=1 << 16)
+
+
+def _names(ba, fn):
+    from boojum_amd.profile import LaunchTrace
+    with LaunchTrace(16) as tr:
+        fn()
+    ba.synchronize()
+    names = list(tr.names)
+    tr.destroy()
+    return names
+
+
+@pytest.mark.parametrize("log_n", [10, 14, 17])
+def test_tile_log_choice(ba, log_n):
+    """tile_log 10 runs the latency-shaped passes (1024-element tiles) at a batch size where the
+    built-in rule would not, tile_log 12 runs the 4096-element tiles at one where it would; both
+    match the oracle in every mode, and tile_log 0 restores the batch-size rule."""
+    n = 1 << log_n
+    big = (1 << (LAT_MAX_LOG_ELEMS + 1) >> log_n) + 1
+    rng = np.random.default_rng(200 + log_n)
+    modes = [(False, False, 1, 1), (True, True, 0, 0), (False, True, 2, 3), (True, False, 3, 5)]
+    for tile, cols, want_lat in ((10, big, True), (12, 2, False), (0, 2, True), (0, big, False)):
+        x = raw_u64(rng, cols * n)
+        for m in modes:
+            bits, reps = ba.tune.get_plan(ba._test_ctx, log_n, m[0])
+            ba.tune.set_plan(ba._test_ctx, log_n, m[0], bits, reps, tile_log=tile)
+            assert ba.tune.get_tile_log(ba._test_ctx, log_n, m[0]) == tile
+            log(f"tile_log={tile} log_n={log_n} cols={cols} mode={m}")
+            d_in = ba.DeviceAllocation.from_host(x)
+            d_out = ba.DeviceAllocation(cols * n)
+            try:
+                names = _names(ba, lambda: ba.ntt.batch_ntt_ctx(ba._test_ctx, d_in.ptr, d_out.ptr, log_n, cols, n, n,
+                                                                m[0], m[1], m[2], m[3]))
+                assert names and all(("ntt_pass_lat" in s) == want_lat for s in names), (tile, cols, names)
+                got = d_out.to_host()
+            finally:
+                d_in.free()
+                d_out.free()
+            want = gl_oracle.batch_ntt(x, log_n, cols, n, n, m[0], m[1], m[2], m[3], nthreads=8)
+            assert np.array_equal(got, want), (tile, cols, m)
+    for br in (False, True):
+        ba.tune.set_plan(ba._test_ctx, log_n, br, None)
+
+
+def test_tile_log_rejects_invalid(ba):
+    t = ba.tune
+    for log_n, tile in ((20, 10), (3, 10), (14, 11), (14, 8)):  # no latency shape above 18 / below 4
+        bits, reps = t.get_plan(ba._test_ctx, log_n)
+        with pytest.raises(ba.BoojumAmdError):
+            t.set_plan(ba._test_ctx, log_n, False, bits, reps, tile_log=tile)
+    assert t.get_tile_log(ba._test_ctx, 20) == 0
+
+
+def test_batch_plan_takes_tuned_tiles(ba):
+    """Batch plans run a merged NTT transform in the latency-shaped tiles only where its size class
+    is tuned to them (tile_log 10), through their grouped kernels; results match the oracle."""
+    log_n, cols = 14, 6
+    n = 1 << log_n
+    rng = np.random.default_rng(77)
+    x = raw_u64(rng, cols * n)
+    d_in = ba.DeviceAllocation.from_host(x)
+    d_out = ba.DeviceAllocation(cols * n)
+    try:
+        for tile, want_lat in ((10, True), (0, False)):
+            for br in (False, True):
+                bits, reps = ba.tune.get_plan(ba._test_ctx, log_n, br)
+                ba.tune.set_plan(ba._test_ctx, log_n, br, bits, reps, tile_log=tile)
+            # three jobs: forward n2b (coset), inverse b2n, forward n2b (coset) again, 2 columns each
+            jobs = [ba.batch.ntt_job(d_in.sub(2 * k * n, 2 * n), d_out.sub(2 * k * n, 2 * n), log_n, 2, n, n,
+                                     bitrev_inputs=(k == 1), inverse=(k == 1), log_extension_degree=0 if k == 1 else 2,
+                                     coset_index=0 if k == 1 else 1) for k in range(3)]
+            plan = ba.batch.BatchPlan(ba._test_ctx, jobs)
+            try:
+                names = _names(ba, lambda: plan.launch())
+            finally:
+                plan.destroy()
+            log(f"batch plan tile_log={tile}: {names}")
+            assert names and any("ntt_pass_lat" in s for s in names) == want_lat, names
+            got = d_out.to_host()
+            for k in range(3):
+                sl = slice(2 * k * n, 2 * (k + 1) * n)
+                le, ci = (0, 0) if k == 1 else (2, 1)
+                want = gl_oracle.batch_ntt(x[sl], log_n, 2, n, n, k == 1, k == 1, le, ci, nthreads=8)
+                assert np.array_equal(got[sl], want), (tile, k)
+    finally:
+        d_in.free()
+        d_out.free()
+        for br in (False, True):
+            ba.tune.set_plan(ba._test_ctx, log_n, br, None)

@@ -20,7 +20,8 @@ class Job(ctypes.Structure):
 
 class TunedPlan(ctypes.Structure):
     """ba_tuned_plan (include/boojum_amd.h)."""
-    _fields_ = [("num_passes", _u32), ("bits", _u32 * 8), ("reps", _u32 * 8), ("contig_waves", _u32)]
+    _fields_ = [("num_passes", _u32), ("bits", _u32 * 8), ("reps", _u32 * 8), ("contig_waves", _u32),
+                ("tile_log", _u32)]
 
 
 class ServeRequest(ctypes.Structure):

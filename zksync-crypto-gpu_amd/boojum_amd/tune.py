@@ -30,7 +30,15 @@ def get_contig_waves(ctx, log_n, bitrev_inputs=False):
     return p.contig_waves
 
 
-def set_plan(ctx, log_n, bitrev_inputs, bits, reps=None, contig_waves=0):
+def get_tile_log(ctx, log_n, bitrev_inputs=False):
+    """Tile shape of the size class's plain batch NTTs: 10 (latency-shaped 1024-element tiles at every
+    batch size), 12 (4096-element tiles at every batch size) or 0 (built-in: by batch size)."""
+    p = TunedPlan()
+    check(lib().ba_tuning_get(ctx.handle, log_n, int(bitrev_inputs), ctypes.byref(p)), "tuning_get")
+    return p.tile_log
+
+
+def set_plan(ctx, log_n, bitrev_inputs, bits, reps=None, contig_waves=0, tile_log=0):
     """Sets the plan of a size class; bits=None restores the built-in plan."""
     if bits is None:
         check(lib().ba_tuning_set(ctx.handle, log_n, int(bitrev_inputs), None), "tuning_set")
@@ -42,6 +50,7 @@ def set_plan(ctx, log_n, bitrev_inputs, bits, reps=None, contig_waves=0):
     for i, r in enumerate((reps or [])[:8]):
         p.reps[i] = r
     p.contig_waves = contig_waves
+    p.tile_log = tile_log
     check(lib().ba_tuning_set(ctx.handle, log_n, int(bitrev_inputs), ctypes.byref(p)), "tuning_set")
 
 

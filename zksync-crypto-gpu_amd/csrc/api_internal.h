@@ -39,6 +39,7 @@ struct BaTuned {
   NttPlan plan;
   uint32_t reps[8];
   uint32_t contig_waves;  // waves per SIMD of the 9..12-bit contiguous pass: 3, 4, or 0 = built-in
+  uint32_t tile_log;      // plain batch NTTs: 10 latency-shaped tiles, 12 4096-element tiles, 0 by batch size
 };
 
 struct ba_context {
@@ -94,18 +95,26 @@ struct BaPassLaunch {
   uint64_t col0;  // first column of the transform this launch covers
 };
 
+// Which transforms may run the latency-shaped passes (1024-element tiles):
+//   BA_LAT_NEVER  LDE transforms (their coset NTTs gather, and their inverse NTT keeps the
+//                 cosets' pass count);
+//   BA_LAT_TUNED  the size class's tuned tile_log is 10 (batch plans: a merged small transform
+//                 runs beside other groups of its level, where the throughput passes measured
+//                 as fast and the extra grouped launches cost more);
+//   BA_LAT_AUTO   that, or else the batch holds at most 2^BA_LAT_MAX_LOG_ELEMS elements and the
+//                 tuned tile_log is not 12 (single batch NTT calls).
+enum BaLatPolicy { BA_LAT_NEVER = 0, BA_LAT_TUNED = 1, BA_LAT_AUTO = 2 };
 // Appends the pass launches of one batch NTT (arguments validated) to `launches`.
-// allow_latency: a batch of at most 2^BA_LAT_MAX_LOG_ELEMS elements runs the latency-shaped
-// passes (single-batch kernels only: callers that group launches leave it false).
 int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint32_t log_n,
                  uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs,
                  bool inverse, uint32_t log_ext, uint32_t coset_idx, bool in_bitrev_gather,
-                 std::vector<BaPassLaunch> &launches, bool allow_latency = false);
+                 std::vector<BaPassLaunch> &launches, BaLatPolicy lat = BA_LAT_NEVER);
 // Per-call time over log_n 10..18 and batches of 2^14..2^22 elements (MI355X): up to 2^20
 // elements the latency passes take 1.0-3.1x less time per call, above it the throughput passes win.
 constexpr uint32_t BA_LAT_MAX_LOG_ELEMS = 20;
-// The batch NTT of num_cols columns of 2^log_n elements runs the latency-shaped passes.
-bool ba_use_latency_passes(uint32_t log_n, uint32_t num_cols);
+// log_n has a latency-shaped plan (4..18); a batch NTT of num_cols columns of 2^log_n runs it.
+bool ba_latency_shape_exists(uint32_t log_n);
+bool ba_use_latency_passes(const ba_context *ctx, uint32_t log_n, bool n2b, uint32_t num_cols, BaLatPolicy lat);
 // Launches them one after another on `stream` (single-batch kernels).
 int ba_run_launches(const std::vector<BaPassLaunch> &launches, hipStream_t stream);
 uint32_t ba_pick_reps(uint64_t tiles, uint32_t cols);

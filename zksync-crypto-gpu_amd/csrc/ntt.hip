@@ -160,16 +160,25 @@ static uint32_t lat_max_log_elems() {
   return v;
 }
 
-bool ba_use_latency_passes(uint32_t log_n, uint32_t num_cols) {
+bool ba_latency_shape_exists(uint32_t log_n) {
   uint32_t bits[2];
+  return lat_bits(log_n, bits) > 0;
+}
+
+bool ba_use_latency_passes(const ba_context *ctx, uint32_t log_n, bool n2b, uint32_t num_cols, BaLatPolicy lat) {
+  if (lat == BA_LAT_NEVER || !ba_latency_shape_exists(log_n)) return false;
+  // a tuned (or caller-set) tile choice for the size class, else the batch-size bound
+  const BaTuned &t = ctx->tuned[n2b][log_n];
+  if (t.valid && t.tile_log) return t.tile_log == LAT_TL;
+  if (lat == BA_LAT_TUNED) return false;
   const uint32_t m = lat_max_log_elems();
-  return m > 0 && lat_bits(log_n, bits) > 0 && ((uint64_t)num_cols << log_n) <= (1ull << std::min(m, 40u));
+  return m > 0 && ((uint64_t)num_cols << log_n) <= (1ull << std::min(m, 40u));
 }
 
 int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint32_t log_n,
                  uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs,
                  bool inverse, uint32_t log_ext, uint32_t coset_idx, bool in_bitrev_gather,
-                 std::vector<BaPassLaunch> &launches, bool allow_latency) {
+                 std::vector<BaPassLaunch> &launches, BaLatPolicy lat) {
   if (num_cols == 0) return BA_SUCCESS;
   const bool n2b = !bitrev_inputs;
   PassArgs base{};
@@ -210,7 +219,7 @@ int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint3
     return BA_SUCCESS;
   }
 
-  if (allow_latency && !in_bitrev_gather && ba_use_latency_passes(log_n, num_cols)) {
+  if (!in_bitrev_gather && ba_use_latency_passes(ctx, log_n, n2b, num_cols, lat)) {
     uint32_t bits[2];
     const int np = lat_bits(log_n, bits);
     for (int p = 0; p < np; p++) {
@@ -324,7 +333,7 @@ int ba_launch_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint
                   bool in_bitrev_gather) {
   std::vector<BaPassLaunch> launches;
   const int rc = ba_build_ntt(ctx, in, out, log_n, num_cols, stride_in, stride_out, bitrev_inputs,
-                              inverse, log_ext, coset_idx, in_bitrev_gather, launches, true);
+                              inverse, log_ext, coset_idx, in_bitrev_gather, launches, BA_LAT_AUTO);
   if (rc) return rc;
   return ba_run_launches(launches, stream);
 }

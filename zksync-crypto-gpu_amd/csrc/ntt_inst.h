@@ -25,8 +25,8 @@ BaKernels ba_pick_contig_hi3(uint32_t S, bool n2b, bool inverse, int scale, bool
 BaKernels ba_pick_strided(uint32_t S, bool n2b, bool inverse, bool top, bool scale, bool gather);
 // whole transform of log_n <= 3 per thread
 BaKernels ba_pick_tiny(uint32_t log_n, bool n2b, bool inverse);
-// latency-shaped passes (ntt_lat.h) for batches too small to fill the chip: contiguous over
-// 4..10 bits, strided over 5..8 bits; no grouped variant (batch plans use the passes above)
+// latency-shaped passes (ntt_lat.h), 1024-element tiles: contiguous over 4..10 bits, strided over
+// 5..8 bits
 constexpr int BA_LAT_MIN_CONTIG = 4, BA_LAT_MAX_CONTIG = 10;
 constexpr int BA_LAT_MIN_STRIDED = 5, BA_LAT_MAX_STRIDED = 8;
 BaKernels ba_pick_lat(uint32_t S, bool strided, bool n2b, bool inverse, bool top, bool scale);

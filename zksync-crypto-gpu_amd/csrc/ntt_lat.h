@@ -183,6 +183,12 @@ template <int S, bool N2B, bool INV, bool STRIDED, bool TOP, bool SCALE>
 __global__ __launch_bounds__(THREADS) void ntt_pass_lat(PassArgs a) {
   lat_body<S, N2B, INV, STRIDED, TOP, SCALE>(a, blockIdx.x);
 }
+// a group of batches (batch plans), as the grouped throughput passes
+template <int S, bool N2B, bool INV, bool STRIDED, bool TOP, bool SCALE>
+__global__ __launch_bounds__(THREADS) void ntt_pass_lat_group(const PassArgs *jobs, const u32 *wg_start, u32 njobs) {
+  const u32 j = group_of(wg_start, njobs, blockIdx.x);
+  lat_body<S, N2B, INV, STRIDED, TOP, SCALE>(jobs[j], blockIdx.x - wg_start[j]);
+}
 
 }  // namespace ntt
 }  // namespace gl

@@ -9,7 +9,7 @@ using namespace gl::ntt;
 
 template <int S, bool N2B, bool INV, bool STRIDED, bool TOP, bool SCALE>
 static BaKernels one() {
-  return {ntt_pass_lat<S, N2B, INV, STRIDED, TOP, SCALE>, nullptr};
+  return {ntt_pass_lat<S, N2B, INV, STRIDED, TOP, SCALE>, ntt_pass_lat_group<S, N2B, INV, STRIDED, TOP, SCALE>};
 }
 
 template <int S, bool N2B, bool INV>

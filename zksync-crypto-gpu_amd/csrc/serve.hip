@@ -107,6 +107,14 @@ constexpr size_t kDescBytes = 16u << 20;  // plan image capacity per slot
 uint64_t work_elems(const ba_serve_request &r) { return (uint64_t)r.num_cols << r.log_n; }
 uint64_t out_elems(const ba_serve_request &r) { return work_elems(r) << r.log_blowup; }
 
+// Size class of a request's output: 0 below 2^26 elements, 1 below 2^30, 2 from 2^30 up.  A batch
+// holds one class: merged into one plan with a request ~10^4 times its size, a small request would
+// complete only when that one does.
+int size_class(const ba_serve_request &r) {
+  const uint64_t e = out_elems(r);
+  return e < (1ull << 26) ? 0 : (e < (1ull << 30) ? 1 : 2);
+}
+
 }  // namespace
 
 struct ba_server {
@@ -196,12 +204,16 @@ void ba_server::fail(Request &r, int status, Clock::time_point now) {
   failed++;
 }
 
-// Enqueues every admitted request (as many as the plan image capacity allows) on slot `s`.
-// Called with the lock held; releases it while planning and enqueueing.  Each request is first
-// validated on its own, so that one client's invalid request (say, device evaluations that
-// overlap the arena) fails only itself and not the requests batched with it.
+// Enqueues the admitted requests of the oldest admitted request's size class (as many as the plan
+// image capacity allows) on slot `s`; the other classes wait for the next free slot.  Called with
+// the lock held; releases it while planning and enqueueing.  Each request is first validated on
+// its own, so that one client's invalid request (say, device evaluations that overlap the arena)
+// fails only itself and not the requests batched with it.
 bool ba_server::launch(Slot &s, std::unique_lock<std::mutex> &lk) {
-  std::vector<uint64_t> take(admitted.begin(), admitted.end());
+  const int cls = size_class(reqs.at(admitted.front()).req);
+  std::vector<uint64_t> take;
+  for (uint64_t t : admitted)
+    if (size_class(reqs.at(t).req) == cls) take.push_back(t);
   std::vector<Request> rs;
   for (uint64_t t : take) rs.push_back(reqs.at(t));
   const uint64_t batch = batches++;
@@ -328,13 +340,12 @@ void ba_server::loop() {
         ++it;
       }
     }
-    if (!admitted.empty()) {
-      for (Slot &s : slots) {
-        if (s.busy) continue;
-        launch(s, lk);
-        busy = true;
-        break;
-      }
+    // one batch per free slot, oldest size class first, while admitted requests remain
+    for (Slot &s : slots) {
+      if (admitted.empty()) break;
+      if (s.busy) continue;
+      launch(s, lk);
+      busy = true;
     }
     if (stop && !busy && admitted.empty() && !queued.empty()) {
       // nothing runs and the head still does not fit: only releases could make room, and a

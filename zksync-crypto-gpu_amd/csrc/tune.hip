@@ -1,6 +1,7 @@
 // tune.hip -- the autotuner: per size class (log_n, input order) it times every pass split, then
 // the columns-per-workgroup of each pass, then the waves per SIMD the contiguous pass runs at,
-// and keeps the fastest on the context.
+// then the tile shape (4096-element tiles of 16 elements per thread against the latency-shaped
+// 1024-element tiles of 4), and keeps the fastest on the context.
 //
 // No reference counterpart: boojum-cuda's kernel shapes per log_n are fixed tables
 // (crates/boojum-cuda/src/ntt.rs:57-176).  The candidate splits are those the pass kernels
@@ -12,6 +13,7 @@
 #include <vector>
 
 #include "api_internal.h"
+#include "ntt_lat.h"
 
 namespace {
 
@@ -100,7 +102,7 @@ int time_entry(ba_context *ctx, const BaTuned &entry, uint32_t log_n, bool n2b, 
   ctx->tuned[n2b][log_n] = entry;
   const uint64_t n = 1ull << log_n;
   std::vector<BaPassLaunch> ls;
-  const int rc = ba_build_ntt(ctx, in, out, log_n, cols, n, n, !n2b, false, 0, 0, false, ls);
+  const int rc = ba_build_ntt(ctx, in, out, log_n, cols, n, n, !n2b, false, 0, 0, false, ls, BA_LAT_AUTO);
   if (rc) return rc;
   return timer.time(ls, ms);
 }
@@ -110,8 +112,10 @@ int tune_class(ba_context *ctx, uint32_t log_n, bool n2b, const uint64_t *in, ui
   const uint32_t cols = (uint32_t)std::max<uint64_t>(1, elems >> log_n);
   BaTuned best{};
   float best_ms = 1e30f;
+  // the 4096-element-tile candidates are timed as such (tile_log 12) whatever the trial size
   for (const auto &split : candidate_splits(log_n)) {
     BaTuned e{};
+    e.tile_log = 12;
     e.valid = ba_plan_from_bits(log_n, n2b, split.data(), (int)split.size(), e.plan);
     if (!e.valid) return BA_ERROR_INVALID_VALUE;  // the enumeration only yields valid splits
     float ms = 0;
@@ -143,6 +147,19 @@ int tune_class(ba_context *ctx, uint32_t log_n, bool n2b, const uint64_t *in, ui
       if (ms < best_ms * 0.99f) best_ms = ms, best = e;
     }
   }
+  // then the latency-shaped passes (1024-element tiles, 4 elements per thread) for every batch
+  // size of the class; if they lose, the class keeps the built-in rule (they still run batches of
+  // at most 2^20 elements, where they measured faster throughout)
+  bool lat = false;
+  if (ba_latency_shape_exists(log_n)) {
+    BaTuned e = best;
+    e.tile_log = gl::ntt::LAT_TL;
+    float ms = 0;
+    const int rc = time_entry(ctx, e, log_n, n2b, in, out, cols, timer, ms);
+    if (rc) return rc;
+    lat = ms < best_ms * 0.99f;
+  }
+  best.tile_log = lat ? gl::ntt::LAT_TL : 0;
   ctx->tuned[n2b][log_n] = best;
   return BA_SUCCESS;
 }
@@ -196,6 +213,7 @@ extern "C" int ba_tuning_get(ba_context_t ctx, uint32_t log_n, int bitrev_inputs
       plan->reps[low] = t.valid ? t.reps[i] : 0;
     }
     plan->contig_waves = t.valid ? t.contig_waves : 0;
+    plan->tile_log = t.valid ? t.tile_log : 0;
     return BA_SUCCESS;
   });
 }
@@ -219,6 +237,10 @@ extern "C" int ba_tuning_set(ba_context_t ctx, uint32_t log_n, int bitrev_inputs
     }
     if (plan->contig_waves != 0 && plan->contig_waves != 3 && plan->contig_waves != 4) return BA_ERROR_INVALID_VALUE;
     e.contig_waves = plan->contig_waves;
+    if (plan->tile_log != 0 && plan->tile_log != 12 &&
+        !(plan->tile_log == (uint32_t)gl::ntt::LAT_TL && ba_latency_shape_exists(log_n)))
+      return BA_ERROR_INVALID_VALUE;
+    e.tile_log = plan->tile_log;
     e.valid = true;
     ctx->tuned[n2b][log_n] = e;
     return BA_SUCCESS;

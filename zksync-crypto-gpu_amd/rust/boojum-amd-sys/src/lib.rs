@@ -55,6 +55,7 @@ pub struct ba_tuned_plan {
     pub bits: [u32; 8],
     pub reps: [u32; 8],
     pub contig_waves: u32,
+    pub tile_log: u32,
 }
 
 #[repr(C)]
