@@ -800,10 +800,11 @@ def main():
                      # per size class and input order: pass split (low bits first), contiguous-pass
                      # waves per SIMD, tile (10: 1024-element latency-shaped tiles at every batch
                      # size, 0: by batch size)
-                     "choices": {f"{ln}{'b' if br else 'n'}": [ba.tune.get_plan(uctx, ln, br)[0],
-                                                             ba.tune.get_contig_waves(uctx, ln, br),
-                                                             ba.tune.get_tile_log(uctx, ln, br)]
-                                 for ln in range(12, 23) for br in (False, True)}}
+                     # (n: natural-input NTTs, b: bit-reversed input, l: the LDE's coset NTTs)
+                     "choices": {f"{ln}{'nbl'[br]}": [ba.tune.get_plan(uctx, ln, br)[0],
+                                                     ba.tune.get_contig_waves(uctx, ln, br),
+                                                     ba.tune.get_tile_log(uctx, ln, br)]
+                                 for ln in range(12, 23) for br in (0, 1, ba.tune.LDE_COSETS)}}
         line["uneven"] = run_uneven(ba, uctx, stream)
         line["uneven"]["tuning"] = utune
         uctx.destroy()

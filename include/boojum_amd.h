@@ -279,7 +279,8 @@ int ba_batch_plan_destroy(ba_batch_plan_t plan);
  * bits (at most 8 passes).
  *
  * ba_autotune times every valid split, then each pass's reps, then contig_waves, then (log_n 4..18)
- * the 1024-element tiles against the best of those, for log_n in [log_n_min,
+ * the 1024-element tiles against the best of those, for plain NTTs of both input orders and for
+ * the LDE's coset NTTs (BA_TUNING_LDE_COSETS below), for log_n in [log_n_min,
  * log_n_max] and both input orders, on batches of max(elems_per_trial, 2^log_n) elements of
  * scratch memory it allocates and frees, and keeps the fastest on `ctx`; every later transform
  * (NTT, LDE, batch plan) on `ctx` uses it.  Results are bit-identical for every plan.  Tuning
@@ -293,6 +294,12 @@ typedef struct {
 } ba_tuned_plan;
 int ba_autotune(ba_context_t ctx, uint32_t log_n_min, uint32_t log_n_max, uint64_t elems_per_trial,
                 ba_stream_t stream);
+/* bitrev_inputs = BA_TUNING_LDE_COSETS addresses the plan of the LDE's coset NTTs (natural ->
+ * bit-reversed forward NTTs that gather the bit-reversed monomials and scale by the coset in their
+ * first pass), which ba_autotune times on that transform: its strided passes are at most 8 bits
+ * and its tile_log is 0.  0 and 1 address plain batch NTTs of natural and bit-reversed input
+ * (the LDE's inverse NTT follows 0). */
+#define BA_TUNING_LDE_COSETS 2
 /* the plan in use for (log_n, bitrev_inputs): tuned if set, else the built-in one */
 int ba_tuning_get(ba_context_t ctx, uint32_t log_n, int bitrev_inputs, ba_tuned_plan *plan);
 /* sets (plan != NULL, validated) or clears (plan == NULL) the tuned plan of a size class */

@@ -151,8 +151,16 @@ def test_set_plan_rejects_invalid(ba):
     for log_n in (0, 29):
         with pytest.raises(ba.BoojumAmdError):
             t.get_plan(ba._test_ctx, log_n)
+    # the LDE's coset NTTs: no strided pass above 8 bits (no gathered variant), no latency tiles
+    with pytest.raises(ba.BoojumAmdError):
+        t.set_plan(ba._test_ctx, 21, t.LDE_COSETS, [12, 9])
+    with pytest.raises(ba.BoojumAmdError):
+        t.set_plan(ba._test_ctx, 14, t.LDE_COSETS, [10, 4], tile_log=10)
+    with pytest.raises(ba.BoojumAmdError):
+        t.get_plan(ba._test_ctx, 14, 3)
     # a rejected plan leaves the size class unchanged
     assert t.get_plan(ba._test_ctx, 20) == ([12, 8], [0, 0])
+    assert t.get_plan(ba._test_ctx, 21, t.LDE_COSETS) == ([12, 5, 4], [0, 0, 0])
 
 
 def test_autotune_then_bit_exact(ba):
@@ -165,6 +173,10 @@ def test_autotune_then_bit_exact(ba):
             assert sum(bits) == log_n and bits in ba.tune.candidate_splits(log_n)
             assert all(r in (0, 1, 2, 4, 8) for r in reps)
             assert ba.tune.get_tile_log(ba._test_ctx, log_n, br) in (0, 10)
+        bits, reps = ba.tune.get_plan(ba._test_ctx, log_n, ba.tune.LDE_COSETS)
+        log(f"tuned log_n={log_n} LDE cosets: bits={bits} reps={reps}")
+        assert bits in ba.tune.candidate_splits(log_n) and all(b <= 8 for b in bits[1:])
+        assert ba.tune.get_tile_log(ba._test_ctx, log_n, ba.tune.LDE_COSETS) == 0
     rng = np.random.default_rng(3)
     for log_n in (10, 13, 17):
         n = 1 << log_n
@@ -277,3 +289,33 @@ def test_batch_plan_takes_tuned_tiles(ba):
         d_out.free()
         for br in (False, True):
             ba.tune.set_plan(ba._test_ctx, log_n, br, None)
+
+
+@pytest.mark.parametrize("log_n,bits", [(14, [9, 5]), (14, [10, 4]), (17, [9, 8]), (20, [12, 4, 4]), (20, [11, 5, 4])])
+def test_lde_coset_plans_bit_exact(ba, log_n, bits):
+    """The LDE's coset NTTs under their own plan entry (gathered first pass of 4..8 bits, coset
+    scale), with reps cycling, at blowups 2 and 4 against the oracle; the plain NTT plan of the
+    class is left alone."""
+    cols = 3
+    n = 1 << log_n
+    x = raw_u64(np.random.default_rng(300 + log_n + bits[0]), cols * n)
+    plain = ba.tune.get_plan(ba._test_ctx, log_n, False)
+    try:
+        for k, lb in enumerate((1, 2)):
+            reps = [(1, 2, 4, 8)[(k + i) % 4] for i in range(len(bits))]
+            ba.tune.set_plan(ba._test_ctx, log_n, ba.tune.LDE_COSETS, bits, reps)
+            assert ba.tune.get_plan(ba._test_ctx, log_n, ba.tune.LDE_COSETS) == (bits, reps)
+            assert ba.tune.get_plan(ba._test_ctx, log_n, False) == plain
+            log(f"LDE coset plan log_n={log_n} bits={bits} reps={reps} blowup={1 << lb}")
+            d_ev = ba.DeviceAllocation.from_host(x)
+            d_out = ba.DeviceAllocation(cols * (n << lb))
+            try:
+                ba.lde.lde_batch(ba._test_ctx, d_ev, log_n, cols, lb, d_out)
+                ba.synchronize()
+                got = d_out.to_host()
+            finally:
+                d_ev.free()
+                d_out.free()
+            assert np.array_equal(got, gl_oracle.lde(x, log_n, cols, lb, nthreads=8)), (bits, lb)
+    finally:
+        ba.tune.set_plan(ba._test_ctx, log_n, ba.tune.LDE_COSETS, None)

@@ -6,6 +6,9 @@ import ctypes
 
 from ._lib import TunedPlan, check, lib
 
+# bitrev_inputs value that addresses the LDE's coset NTTs' plan (BA_TUNING_LDE_COSETS)
+LDE_COSETS = 2
+
 
 def autotune(ctx, log_n_min, log_n_max, elems_per_trial=1 << 26, stream=None):
     """Times every valid split and per-pass reps for each size class in the range; the fastest

@@ -49,7 +49,9 @@ struct ba_context {
   void *mem;
   size_t bytes;
   gl::Tables tab;
-  BaTuned tuned[2][BA_MAX_LOG_N + 1] = {};  // [n2b][log_n]
+  // [kind][log_n]: kind 0 plain b2n, 1 plain n2b (and the LDE's inverse NTT), 2 the LDE's
+  // gathered coset NTTs (BA_TUNING_LDE_COSETS)
+  BaTuned tuned[3][BA_MAX_LOG_N + 1] = {};
   // batch plans and servers hold pointers into the tables: the context outlives them
   std::atomic<int> users{0};
   // Poseidon2 round constants (30 x 12, device memory), installed by the caller

@@ -252,17 +252,12 @@ int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint3
     return BA_SUCCESS;
   }
 
-  // The gathered first pass of an LDE's coset NTT has no 9- or 10-bit variant (the gathered and
-  // scaled wide pass measured 11 % slower than the built-in split at 2^21): a tuned plan with such
-  // a pass gives way to the built-in plan there.
-  bool tuned = ctx->tuned[n2b][log_n].valid;
-  if (tuned && in_bitrev_gather) {
-    const NttPlan &tp = ctx->tuned[n2b][log_n].plan;
-    for (int p = 0; p < tp.num_passes; p++)
-      if (!tp.contig[p] && tp.bits[p] > 8) tuned = false;
-  }
-  const BaTuned &tuning = tuned ? ctx->tuned[n2b][log_n] : BaTuned{};
-  const NttPlan plan = tuned ? tuning.plan : ba_plan_ntt(log_n, n2b);
+  // The size class's tuned plan, else the built-in one.  An LDE's coset NTTs (the gathered n2b
+  // forward) have their own entry, timed on that transform: a plan tuned on plain NTTs does not
+  // carry over (the gathered, scaled first pass has other costs; it has no 9- or 10-bit variant).
+  const int kind = in_bitrev_gather ? BA_TUNING_LDE_COSETS : (n2b ? 1 : 0);
+  const BaTuned &tuning = ctx->tuned[kind][log_n].valid ? ctx->tuned[kind][log_n] : BaTuned{};
+  const NttPlan plan = tuning.valid ? tuning.plan : ba_plan_ntt(log_n, n2b);
   for (int p = 0; p < plan.num_passes; p++) {
     const uint32_t S = plan.bits[p];
     const bool contig = plan.contig[p];

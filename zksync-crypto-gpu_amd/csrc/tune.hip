@@ -97,29 +97,43 @@ struct Timer {
   }
 };
 
-int time_entry(ba_context *ctx, const BaTuned &entry, uint32_t log_n, bool n2b, const uint64_t *in, uint64_t *out,
+// kind: 0 plain b2n forward, 1 plain n2b forward, BA_TUNING_LDE_COSETS the LDE's coset NTT (n2b
+// forward with coset scale, reading its input bit-reversed through the gathered first pass)
+int time_entry(ba_context *ctx, const BaTuned &entry, uint32_t log_n, int kind, const uint64_t *in, uint64_t *out,
                uint32_t cols, Timer &timer, float &ms) {
-  ctx->tuned[n2b][log_n] = entry;
+  ctx->tuned[kind][log_n] = entry;
   const uint64_t n = 1ull << log_n;
   std::vector<BaPassLaunch> ls;
-  const int rc = ba_build_ntt(ctx, in, out, log_n, cols, n, n, !n2b, false, 0, 0, false, ls, BA_LAT_AUTO);
+  const bool lde = kind == BA_TUNING_LDE_COSETS;
+  const int rc = lde ? ba_build_ntt(ctx, in, out, log_n, cols, n, n, false, false, 1, 1, true, ls, BA_LAT_NEVER)
+                     : ba_build_ntt(ctx, in, out, log_n, cols, n, n, kind == 0, false, 0, 0, false, ls, BA_LAT_AUTO);
   if (rc) return rc;
   return timer.time(ls, ms);
 }
 
-int tune_class(ba_context *ctx, uint32_t log_n, bool n2b, const uint64_t *in, uint64_t *out, uint64_t elems,
+// the gathered first pass (top strided pass) of an LDE coset NTT exists for 4..8 bits
+bool lde_split(const std::vector<uint32_t> &split) {
+  for (size_t i = 1; i < split.size(); i++)
+    if (split[i] > 8) return false;
+  return true;
+}
+
+int tune_class(ba_context *ctx, uint32_t log_n, int kind, const uint64_t *in, uint64_t *out, uint64_t elems,
                Timer &timer) {
+  const bool n2b = kind != 0;
+  const bool lde = kind == BA_TUNING_LDE_COSETS;
   const uint32_t cols = (uint32_t)std::max<uint64_t>(1, elems >> log_n);
   BaTuned best{};
   float best_ms = 1e30f;
   // the 4096-element-tile candidates are timed as such (tile_log 12) whatever the trial size
   for (const auto &split : candidate_splits(log_n)) {
+    if (lde && !lde_split(split)) continue;
     BaTuned e{};
     e.tile_log = 12;
     e.valid = ba_plan_from_bits(log_n, n2b, split.data(), (int)split.size(), e.plan);
     if (!e.valid) return BA_ERROR_INVALID_VALUE;  // the enumeration only yields valid splits
     float ms = 0;
-    const int rc = time_entry(ctx, e, log_n, n2b, in, out, cols, timer, ms);
+    const int rc = time_entry(ctx, e, log_n, kind, in, out, cols, timer, ms);
     if (rc) return rc;
     if (ms < best_ms) best_ms = ms, best = e;
   }
@@ -129,7 +143,7 @@ int tune_class(ba_context *ctx, uint32_t log_n, bool n2b, const uint64_t *in, ui
       BaTuned e = best;
       e.reps[p] = r;
       float ms = 0;
-      const int rc = time_entry(ctx, e, log_n, n2b, in, out, cols, timer, ms);
+      const int rc = time_entry(ctx, e, log_n, kind, in, out, cols, timer, ms);
       if (rc) return rc;
       if (ms < best_ms * 0.99f) best_ms = ms, best = e;
     }
@@ -142,7 +156,7 @@ int tune_class(ba_context *ctx, uint32_t log_n, bool n2b, const uint64_t *in, ui
       BaTuned e = best;
       e.contig_waves = w;
       float ms = 0;
-      const int rc = time_entry(ctx, e, log_n, n2b, in, out, cols, timer, ms);
+      const int rc = time_entry(ctx, e, log_n, kind, in, out, cols, timer, ms);
       if (rc) return rc;
       if (ms < best_ms * 0.99f) best_ms = ms, best = e;
     }
@@ -151,16 +165,16 @@ int tune_class(ba_context *ctx, uint32_t log_n, bool n2b, const uint64_t *in, ui
   // size of the class; if they lose, the class keeps the built-in rule (they still run batches of
   // at most 2^20 elements, where they measured faster throughout)
   bool lat = false;
-  if (ba_latency_shape_exists(log_n)) {
+  if (!lde && ba_latency_shape_exists(log_n)) {
     BaTuned e = best;
     e.tile_log = gl::ntt::LAT_TL;
     float ms = 0;
-    const int rc = time_entry(ctx, e, log_n, n2b, in, out, cols, timer, ms);
+    const int rc = time_entry(ctx, e, log_n, kind, in, out, cols, timer, ms);
     if (rc) return rc;
     lat = ms < best_ms * 0.99f;
   }
   best.tile_log = lat ? gl::ntt::LAT_TL : 0;
-  ctx->tuned[n2b][log_n] = best;
+  ctx->tuned[kind][log_n] = best;
   return BA_SUCCESS;
 }
 
@@ -182,13 +196,14 @@ extern "C" int ba_autotune(ba_context_t ctx, uint32_t log_n_min, uint32_t log_n_
     hipError_t err = hipMalloc(&in, elems * 8);
     if (err == hipSuccess) err = hipMalloc(&out, elems * 8);
     int rc = (int)err;
-    const auto saved = std::vector<BaTuned>(&ctx->tuned[0][0], &ctx->tuned[0][0] + 2 * (BA_MAX_LOG_N + 1));
+    const auto saved = std::vector<BaTuned>(&ctx->tuned[0][0], &ctx->tuned[0][0] + 3 * (BA_MAX_LOG_N + 1));
     if (rc == BA_SUCCESS) rc = ba_fill_uniform(in, elems, elems, 1, 0x7475u, stream);
     {
       Timer timer((hipStream_t)stream);
       if (rc == BA_SUCCESS) rc = timer.init();
       for (uint32_t log_n = log_n_min; rc == BA_SUCCESS && log_n <= log_n_max; log_n++)
-        for (int n2b = 1; rc == BA_SUCCESS && n2b >= 0; n2b--) rc = tune_class(ctx, log_n, n2b != 0, in, out, elems, timer);
+        for (int kind : {1, 0, BA_TUNING_LDE_COSETS})
+          if (rc == BA_SUCCESS) rc = tune_class(ctx, log_n, kind, in, out, elems, timer);
     }
     if (rc != BA_SUCCESS) std::copy(saved.begin(), saved.end(), &ctx->tuned[0][0]);  // keep the old state
     (void)hipStreamSynchronize((hipStream_t)stream);
@@ -201,9 +216,11 @@ extern "C" int ba_autotune(ba_context_t ctx, uint32_t log_n_min, uint32_t log_n_
 extern "C" int ba_tuning_get(ba_context_t ctx, uint32_t log_n, int bitrev_inputs, ba_tuned_plan *plan) {
   return ba_guarded([&]() -> int {
     if (!ctx) return BA_ERROR_NOT_INITIALIZED;
-    if (!plan || log_n < 1 || log_n > BA_MAX_LOG_N) return BA_ERROR_INVALID_VALUE;
-    const bool n2b = !bitrev_inputs;
-    const BaTuned &t = ctx->tuned[n2b][log_n];
+    if (!plan || log_n < 1 || log_n > BA_MAX_LOG_N || bitrev_inputs < 0 || bitrev_inputs > BA_TUNING_LDE_COSETS)
+      return BA_ERROR_INVALID_VALUE;
+    const int kind = bitrev_inputs == BA_TUNING_LDE_COSETS ? BA_TUNING_LDE_COSETS : (bitrev_inputs ? 0 : 1);
+    const bool n2b = kind != 0;
+    const BaTuned &t = ctx->tuned[kind][log_n];
     const NttPlan p = t.valid ? t.plan : ba_plan_ntt(log_n, n2b);
     *plan = ba_tuned_plan{};
     plan->num_passes = (uint32_t)p.num_passes;
@@ -221,10 +238,12 @@ extern "C" int ba_tuning_get(ba_context_t ctx, uint32_t log_n, int bitrev_inputs
 extern "C" int ba_tuning_set(ba_context_t ctx, uint32_t log_n, int bitrev_inputs, const ba_tuned_plan *plan) {
   return ba_guarded([&]() -> int {
     if (!ctx) return BA_ERROR_NOT_INITIALIZED;
-    if (log_n < 1 || log_n > BA_MAX_LOG_N) return BA_ERROR_INVALID_VALUE;
-    const bool n2b = !bitrev_inputs;
+    if (log_n < 1 || log_n > BA_MAX_LOG_N || bitrev_inputs < 0 || bitrev_inputs > BA_TUNING_LDE_COSETS)
+      return BA_ERROR_INVALID_VALUE;
+    const int kind = bitrev_inputs == BA_TUNING_LDE_COSETS ? BA_TUNING_LDE_COSETS : (bitrev_inputs ? 0 : 1);
+    const bool n2b = kind != 0;
     if (!plan) {
-      ctx->tuned[n2b][log_n] = BaTuned{};
+      ctx->tuned[kind][log_n] = BaTuned{};
       return BA_SUCCESS;
     }
     if (plan->num_passes < 1 || plan->num_passes > 8) return BA_ERROR_INVALID_VALUE;
@@ -241,8 +260,13 @@ extern "C" int ba_tuning_set(ba_context_t ctx, uint32_t log_n, int bitrev_inputs
         !(plan->tile_log == (uint32_t)gl::ntt::LAT_TL && ba_latency_shape_exists(log_n)))
       return BA_ERROR_INVALID_VALUE;
     e.tile_log = plan->tile_log;
+    if (kind == BA_TUNING_LDE_COSETS) {  // no latency-shaped tiles, no gathered pass above 8 bits
+      if (e.tile_log) return BA_ERROR_INVALID_VALUE;
+      for (uint32_t i = 1; i < plan->num_passes; i++)
+        if (plan->bits[i] > 8) return BA_ERROR_INVALID_VALUE;
+    }
     e.valid = true;
-    ctx->tuned[n2b][log_n] = e;
+    ctx->tuned[kind][log_n] = e;
     return BA_SUCCESS;
   });
 }

@@ -18,6 +18,7 @@ pub const BA_MAX_LOG_N: u32 = 28;
 pub const BA_JOB_NTT: u32 = 0;
 pub const BA_JOB_LDE: u32 = 1;
 pub const BA_LDE_NATURAL_MONOMIALS: u32 = 1;
+pub const BA_TUNING_LDE_COSETS: c_int = 2;
 pub const BA_SERVE_INPUT_HOST: u32 = 0;
 pub const BA_SERVE_INPUT_DEVICE: u32 = 1;
 pub const BA_SERVE_INPUT_SYNTHETIC: u32 = 2;
