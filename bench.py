@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-per-call", action="store_true", help="skip the per-column-call leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch leg")
-    ap.add_argument("--no-tune", action="store_true", help="uneven leg: built-in pass plans, no autotuning")
+    ap.add_argument("--no-tune", action="store_true", help="uneven and serve legs: built-in pass plans, no autotuning")
     ap.add_argument("--tune-headline", action="store_true",
                     help="autotune the headline size class too (default: built-in plan, which the tuner "
                          "keeps for 2^20)")
@@ -821,10 +821,20 @@ def main():
     if not args.no_serve:
         # the default policy (later requests that fit pass a head waiting for memory, at most 32 per
         # head, two slots), then strict arrival order and four slots on the same request stream
-        line["serve"] = run_serve(ba, ctx)
+        # on the server's own context, autotuned over the stream's size classes before the first
+        # request (as a server would at start-up; tuning runs apart from every timed region)
+        sctx = ba.Context.create(12, 12)
+        stune = None
+        if not args.no_tune:
+            t0 = time.perf_counter()
+            ba.tune.autotune(sctx, 12, 23, elems_per_trial=1 << 26)
+            stune = {"log_n_range": [12, 23], "seconds": round(time.perf_counter() - t0, 2)}
+        line["serve"] = run_serve(ba, sctx)
+        line["serve"]["tuning"] = stune
         line["serve"]["variants"] = {
-            "strict_order_2_slots": run_serve(ba, ctx, max_bypass=0),
-            "bypass_4_slots": run_serve(ba, ctx, slots=4)}
+            "strict_order_2_slots": run_serve(ba, sctx, max_bypass=0),
+            "bypass_4_slots": run_serve(ba, sctx, slots=4)}
+        sctx.destroy()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(log_n, lb)
     ba.check(lib.ba_graph_destroy(graph), "graph_destroy")

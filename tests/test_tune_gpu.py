@@ -291,11 +291,13 @@ def test_batch_plan_takes_tuned_tiles(ba):
             ba.tune.set_plan(ba._test_ctx, log_n, br, None)
 
 
-@pytest.mark.parametrize("log_n,bits", [(14, [9, 5]), (14, [10, 4]), (17, [9, 8]), (20, [12, 4, 4]), (20, [11, 5, 4])])
+@pytest.mark.parametrize("log_n,bits", [(14, [9, 5]), (14, [10, 4]), (17, [9, 8]), (17, [9, 4, 4]), (20, [12, 4, 4]),
+                                         (20, [11, 5, 4])])
 def test_lde_coset_plans_bit_exact(ba, log_n, bits):
     """The LDE's coset NTTs under their own plan entry (gathered first pass of 4..8 bits, coset
     scale), with reps cycling, at blowups 2 and 4 against the oracle; the plain NTT plan of the
-    class is left alone."""
+    class is left alone.  With three coset passes under a two-pass inverse NTT (17: [9, 4, 4]
+    under [12, 5]) the cosets still fork onto the auxiliary streams."""
     cols = 3
     n = 1 << log_n
     x = raw_u64(np.random.default_rng(300 + log_n + bits[0]), cols * n)

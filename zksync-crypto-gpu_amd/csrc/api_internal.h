@@ -160,7 +160,7 @@ int ba_validate_lde(uint64_t *evals, uint64_t in_stride, uint64_t *monomials, ui
 int ba_build_lde(const ba_context *ctx, uint64_t *evals, uint64_t in_stride, uint64_t *monomials,
                  uint64_t *out, uint64_t out_col_stride, uint64_t out_coset_stride, uint32_t log_n,
                  uint32_t num_cols, uint32_t log_blowup, std::vector<BaLeveledLaunch> &launches,
-                 uint32_t first_level);
+                 uint32_t first_level, size_t *inverse_launches = nullptr);
 int ba_validate_ntt(uint32_t log_n, uint64_t stride_in, uint64_t stride_out, uint32_t log_ext,
                     uint32_t coset_idx);
 

@@ -93,7 +93,7 @@ int ba_validate_lde(uint64_t *evals, uint64_t in_stride, uint64_t *monomials, ui
 int ba_build_lde(const ba_context *ctx, uint64_t *evals, uint64_t in_stride, uint64_t *monomials,
                  uint64_t *out, uint64_t out_col_stride, uint64_t out_coset_stride, uint32_t log_n,
                  uint32_t num_cols, uint32_t log_blowup, std::vector<BaLeveledLaunch> &launches,
-                 uint32_t first_level) {
+                 uint32_t first_level, size_t *inverse_launches) {
   if (num_cols == 0) return BA_SUCCESS;
   const uint64_t n = 1ull << log_n;
   const uint32_t blowup = 1u << log_blowup;
@@ -108,6 +108,7 @@ int ba_build_lde(const ba_context *ctx, uint64_t *evals, uint64_t in_stride, uin
     launches.push_back({first_level + (uint32_t)l.pass, l});
     depth = std::max<uint32_t>(depth, (uint32_t)l.pass + 1);
   }
+  if (inverse_launches) *inverse_launches = ls.size();
   for (uint32_t c = 0; c < blowup; c++) {
     ls.clear();
     rc = ba_build_ntt(ctx, mono, out + (uint64_t)c * out_coset_stride, log_n, num_cols, mono_stride,
@@ -147,26 +148,28 @@ static int lde_run(ba_context_t ctx, uint64_t *evals, uint64_t in_stride, uint64
     int device = -1;  // the context's tables, and its auxiliary streams, live on its device
     if (hipGetDevice(&device) != hipSuccess || device != ctx->device) return BA_ERROR_INVALID_VALUE;
     std::vector<BaLeveledLaunch> leveled;
+    size_t n_inv = 0;  // the inverse NTT's launches, first in the list
     rc = ba_build_lde(ctx, evals, in_stride, monomials, out, out_col_stride, out_coset_stride, log_n,
-                      num_cols, log_blowup, leveled, 0);
+                      num_cols, log_blowup, leveled, 0, &n_inv);
     if (rc) return rc;
     std::vector<BaPassLaunch> flat;
     for (const BaLeveledLaunch &l : leveled) flat.push_back(l.launch);  // already in dependency order
-    // flat = the inverse NTT's passes, then each coset's: all the same count (one n2b plan per
-    // size class).  The coset NTTs only read the monomials, so cosets 1.. run on auxiliary streams
-    // forked after the inverse NTT and joined back: a pass's last partial wave of workgroups then
-    // overlaps another coset's passes (a graph captured from `stream` holds the same fork/join).
+    // flat = the inverse NTT's passes (their own plan), then each coset's (one plan shared by the
+    // cosets, which may have another pass count).  The coset NTTs only read the monomials, so
+    // cosets 1.. run on auxiliary streams forked after the inverse NTT and joined back: a pass's
+    // last partial wave of workgroups then overlaps another coset's passes (a graph captured from
+    // `stream` holds the same fork/join).
     const uint32_t blowup = 1u << log_blowup;
-    const size_t per = flat.size() / (1 + blowup);
+    const size_t per = (flat.size() - n_inv) / blowup;
     // (under launch tracing the launches stay in order on one stream, so each one's timing is its own)
-    if (blowup == 1 || per == 0 || per * (1 + blowup) != flat.size() || ba_trace_active() || !ctx->lde_fork ||
-        lde_serial())
+    if (blowup == 1 || per == 0 || n_inv == 0 || n_inv + per * blowup != flat.size() || ba_trace_active() ||
+        !ctx->lde_fork || lde_serial())
       return ba_run_launches(flat, (hipStream_t)stream);
     const hipStream_t main = (hipStream_t)stream;
     std::lock_guard<std::mutex> lock(ctx->lde_mutex);
     hipError_t err = hipSuccess;
     const int aux = (int)std::min<uint32_t>(blowup - 1, ba_context::kLdeAux);
-    rc = ba_run_launches(std::vector<BaPassLaunch>(flat.begin(), flat.begin() + per), main);
+    rc = ba_run_launches(std::vector<BaPassLaunch>(flat.begin(), flat.begin() + n_inv), main);
     if (rc) return rc;
     if ((err = hipEventRecord(ctx->lde_fork, main)) != hipSuccess) return (int)err;
     // Once forked, every auxiliary stream that waited on the fork is joined back even when a later
@@ -178,7 +181,7 @@ static int lde_run(ba_context_t ctx, uint64_t *evals, uint64_t in_stride, uint64
     if (rc) forked--;  // the failed wait did not fork that stream
     for (uint32_t c = 0; c < blowup && !rc; c++) {
       const hipStream_t sc = c == 0 ? main : ctx->lde_aux[(c - 1) % aux];
-      const auto first = flat.begin() + per * (1 + c);
+      const auto first = flat.begin() + n_inv + per * c;
       rc = ba_run_launches(std::vector<BaPassLaunch>(first, first + per), sc);
     }
     for (int i = 0; i < forked; i++) {
