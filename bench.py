@@ -115,9 +115,10 @@ def cpu_baseline(log_n, log_blowup):
     }
 
 
-def run_single(ba, ctx, stream, log_n=16, calls=200):
-    """One column of 2^16, natural -> bit-reversed forward NTT, one library call per transform
-    (the reference's CPU-runnable case), inputs resident in HBM."""
+def time_calls(ba, ctx, stream, log_n, calls):
+    """(us per call of back-to-back one-column forward NTT calls, HIP events around them; kernel
+    launches per call, counted by the library's launch tracing on one untimed call)."""
+    from boojum_amd.profile import LaunchTrace
     n = 1 << log_n
     d_in, d_out = ba.DeviceAllocation(n), ba.DeviceAllocation(n)
     try:
@@ -134,11 +135,27 @@ def run_single(ba, ctx, stream, log_n=16, calls=200):
         us = e0.elapsed_ms(e1) / calls * 1e3
         e0.destroy()
         e1.destroy()
+        with LaunchTrace(16) as tr:
+            ba.ntt.batch_ntt_ctx(ctx, d_in.ptr, d_out.ptr, log_n, 1, n, n, False, False, stream=stream)
+        stream.synchronize()
+        launches = tr.count
+        tr.destroy()
     finally:
         d_in.free()
         d_out.free()
+    return us, launches
+
+
+def run_single(ba, ctx, stream, log_n=16, calls=200):
+    """One column of 2^16, natural -> bit-reversed forward NTT, one library call per transform
+    (the reference's CPU-runnable case), inputs resident in HBM.  Beside it, the same for one
+    column of 2^8, which is one launch: the per-call floor a one-launch call pays."""
+    us, launches = time_calls(ba, ctx, stream, log_n, calls)
+    floor, floor_launches = time_calls(ba, ctx, stream, 8, calls)
     return {"workload": f"1 column x 2^{log_n} forward NTT, natural in, bit-reversed out, one call each",
-            "us_per_call": round(us, 2), "melem_s": round(n / us, 1)}
+            "us_per_call": round(us, 2), "melem_s": round((1 << log_n) / us, 1), "launches_per_call": launches,
+            "floor": {"workload": "1 column x 2^8, the same call", "launches_per_call": floor_launches,
+                      "us_per_call": round(floor, 2)}}
 
 
 def uneven_jobs(ba, seed=3, jobs_n=512):
