@@ -236,6 +236,7 @@ int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint3
       a.num_cols = num_cols;
       a.lo_bit = lo;
       a.reps = 1;
+      a.in_canonical = p > 0;
       a.scale = (pre && p == 0) ? SCALE_PRE : ((inverse && p == np - 1) ? SCALE_POST : SCALE_NONE);
       const bool scale = a.scale != SCALE_NONE;
       const uint64_t chunks = (uint64_t)num_cols << (log_n - S);
@@ -289,6 +290,9 @@ int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint3
       a.scale = (pre && p == 0) ? SCALE_PRE
                                 : ((inverse && p == plan.num_passes - 1) ? SCALE_POST : SCALE_NONE);
       a.in_bitrev = in_bitrev_gather && p == 0;
+      // later passes read this transform's own output, and the gathered LDE coset pass reads the
+      // monomials the inverse NTT wrote: values < p
+      a.in_canonical = p > 0 || in_bitrev_gather;
       const uint64_t groups = big ? (nc + a.reps - 1) / a.reps : (tiles + a.reps - 1) / a.reps;
       l.grid = groups << log_stride;
       const bool top = plan.lo_bit[p] + S == log_n;

@@ -269,6 +269,8 @@ struct PassArgs {
   u32 scale_coset;       // the factor depends on j (log_extension_degree > 0)
   u32 coset_mul;         // s^j = 7^j omega_{2^30}^(j * coset_mul)
   u32 in_bitrev;         // first n2b pass: natural element j is read from position bitrev_L(j)
+  u32 in_canonical;      // the input was written by a pass of this library (values < p): loads
+                         // skip the reduction that a caller's raw u64 input needs
   // column k starts at in_cols[k] / out_cols[k] when these lists are given (columns gathered
   // from separate buffers), else at in + k * stride_in / out + k * stride_out
   const u64 *const *in_cols;
@@ -435,11 +437,17 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
     u64 *out = col_out(a, col);
     u64 v[VALS];
     if constexpr (gather) {
-      static_for<VALS>([&](auto k_) {
-        constexpr int k = decltype(k_)::value;
-        const u32 e = t + THREADS * k;
-        lds[(e & ((1u << S) - 1)) * ROW + (e >> S)] = canon(nxt[k]);
-      });
+      auto stage = [&](auto reduce_) {
+        static_for<VALS>([&](auto k_) {
+          constexpr int k = decltype(k_)::value;
+          const u32 e = t + THREADS * k;
+          lds[(e & ((1u << S) - 1)) * ROW + (e >> S)] = decltype(reduce_)::value ? canon(nxt[k]) : nxt[k];
+        });
+      };
+      if (a.in_canonical)
+        stage(std::false_type{});
+      else
+        stage(std::true_type{});
       __syncthreads();
       static_for<VALS>([&](auto m_) {
         constexpr int m = decltype(m_)::value;
@@ -447,10 +455,17 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
       });
       __syncthreads();
     } else {
-      static_for<VALS>([&](auto m_) {
-        constexpr int m = decltype(m_)::value;
-        v[m] = canon(nxt[m]);
-      });
+      if (a.in_canonical) {
+        static_for<VALS>([&](auto m_) {
+          constexpr int m = decltype(m_)::value;
+          v[m] = nxt[m];
+        });
+      } else {
+        static_for<VALS>([&](auto m_) {
+          constexpr int m = decltype(m_)::value;
+          v[m] = canon(nxt[m]);
+        });
+      }
     }
     if (col + 1 < col_end) fetch(col + 1);
     if constexpr (N2B && SCALE) mul_all<true>(v, [&](int m) { return fac[m]; });
@@ -606,15 +621,21 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
     if constexpr (!PREFETCH) fetch(tile);
     // keeps the 16 factors from being hoisted out of the loop (into 32 live registers)
     if constexpr (SCALE && AT_USE) asm volatile("" : "+v"(fbase));
-    static_for<VALS>([&](auto k_) {
-      constexpr int k = decltype(k_)::value;
-      const u32 e = t + THREADS * k;
-      // lds_pad(t + 256 k) = lds_pad(t) + 272 k; a bit-reversed input (single pass) is a
-      // permutation inside each column
-      u32 slot = lds_pad(t) + (THREADS + THREADS / 16) * k;
-      if (N2B && a.in_bitrev) slot = lds_pad(((e >> S) << S) + bitrev32(e & ((1u << S) - 1), L));
-      lds[slot] = canon(nxt[k]);
-    });
+    auto stage = [&](auto reduce_) {
+      static_for<VALS>([&](auto k_) {
+        constexpr int k = decltype(k_)::value;
+        const u32 e = t + THREADS * k;
+        // lds_pad(t + 256 k) = lds_pad(t) + 272 k; a bit-reversed input (single pass) is a
+        // permutation inside each column
+        u32 slot = lds_pad(t) + (THREADS + THREADS / 16) * k;
+        if (N2B && a.in_bitrev) slot = lds_pad(((e >> S) << S) + bitrev32(e & ((1u << S) - 1), L));
+        lds[slot] = decltype(reduce_)::value ? canon(nxt[k]) : nxt[k];
+      });
+    };
+    if (a.in_canonical)
+      stage(std::false_type{});
+    else
+      stage(std::true_type{});
     if (PREFETCH && i + 1 < reps) fetch(tile + (1ull << a.log_tile_stride));
     __syncthreads();
     u64 v[VALS];

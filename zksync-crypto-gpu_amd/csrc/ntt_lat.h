@@ -105,7 +105,8 @@ __device__ __forceinline__ void lat_body(const PassArgs &a, const u32 bid) {
   constexpr int WB0 = Sh::WB(0);
   static_for<4>([&](auto m_) {
     constexpr int m = decltype(m_)::value;
-    v[m] = valid ? canon(gload(in + pos(window2<WB0>(q, m)))) : 0;
+    const u64 x = valid ? gload(in + pos(window2<WB0>(q, m))) : 0;
+    v[m] = a.in_canonical ? x : canon(x);
   });
   u64 tw[R][4];
   static_for<R>([&](auto r_) {
