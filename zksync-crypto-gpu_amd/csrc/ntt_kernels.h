@@ -614,6 +614,12 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
       tile0 < a.tile_count
           ? (u32)min((u64)a.reps, (a.tile_count - tile0 + (1ull << a.log_tile_stride) - 1) >> a.log_tile_stride)
           : 0;
+  // An n2b pass over whole 4096-element tiles starts with the window of the top 4 bits, whose 16
+  // elements per thread (t + 256 m) are the ones the thread loaded: they go straight into
+  // registers, with no LDS staging and its barrier (unless the load is a bit-reversed gather).
+  // The twiddle table's fill then needs a barrier of its own before the first round reads it.
+  const bool direct0 = N2B && S == TILE_LOG && WHOLE && !a.in_bitrev;
+  if (direct0) __syncthreads();
   if (PREFETCH && reps) fetch(tile0);
   for (u32 i = 0; i < reps; i++) {
     const u64 tile = tile0 + ((u64)i << a.log_tile_stride);
@@ -621,6 +627,7 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
     if constexpr (!PREFETCH) fetch(tile);
     // keeps the 16 factors from being hoisted out of the loop (into 32 live registers)
     if constexpr (SCALE && AT_USE) asm volatile("" : "+v"(fbase));
+    u64 v[VALS];
     auto stage = [&](auto reduce_) {
       static_for<VALS>([&](auto k_) {
         constexpr int k = decltype(k_)::value;
@@ -632,13 +639,25 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
         lds[slot] = decltype(reduce_)::value ? canon(nxt[k]) : nxt[k];
       });
     };
-    if (a.in_canonical)
+    if (direct0) {
+      if (a.in_canonical) {
+        static_for<VALS>([&](auto m_) {
+          constexpr int m = decltype(m_)::value;
+          v[m] = nxt[m];
+        });
+      } else {
+        static_for<VALS>([&](auto m_) {
+          constexpr int m = decltype(m_)::value;
+          v[m] = canon(nxt[m]);
+        });
+      }
+    } else if (a.in_canonical) {
       stage(std::false_type{});
-    else
+    } else {
       stage(std::true_type{});
+    }
     if (PREFETCH && i + 1 < reps) fetch(tile + (1ull << a.log_tile_stride));
-    __syncthreads();
-    u64 v[VALS];
+    if (!direct0) __syncthreads();
     static_for<R>([&](auto r_) {
       constexpr int r = decltype(r_)::value;
       constexpr int WB = Sh::WB(r);
@@ -649,11 +668,13 @@ __device__ __forceinline__ void contig_body(const PassArgs &a, const u32 bid) {
       constexpr bool LINEAR = WB == 0 || WB >= 4;
       constexpr u32 STEP = (1u << WB) + (WB >= 4 ? (1u << (WB >= 4 ? WB - 4 : 0)) : 0);
       auto slot = [&](int m) { return LINEAR ? l0 + m * STEP : lds_pad(u0 + ((u32)m << WB)); };
-      static_for<VALS>([&](auto m_) {
-        constexpr int m = decltype(m_)::value;
-        v[m] = lds[slot(m)];
-      });
-      __syncthreads();
+      if (r != 0 || !direct0) {
+        static_for<VALS>([&](auto m_) {
+          constexpr int m = decltype(m_)::value;
+          v[m] = lds[slot(m)];
+        });
+        __syncthreads();
+      }
       if (SC != 0 && !INV && r == 0) scale_all(v);
       if constexpr (WB >= 4) {
         constexpr int K = Sh::K(r), QB = S - 4 - WB;
