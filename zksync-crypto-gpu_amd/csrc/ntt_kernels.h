@@ -361,6 +361,14 @@ __device__ __forceinline__ const u64 *at(const u64 *base, u32 byte_off) {
   return (const u64 *)((const char *)base + byte_off);
 }
 __device__ __forceinline__ u64 *at(u64 *base, u32 byte_off) { return (u64 *)((char *)base + byte_off); }
+// a pointer every lane of the workgroup holds, moved to scalar registers
+template <typename T>
+__device__ __forceinline__ T *in_sgprs(T *p) {
+  const u64 x = (u64)p;
+  const u32 lo = __builtin_amdgcn_readfirstlane((u32)x);
+  const u32 hi = __builtin_amdgcn_readfirstlane((u32)(x >> 32));
+  return (T *)(((u64)hi << 32) | lo);
+}
 
 // TOP: the pass holds the index's top bits (P0 + S == L), so one round has no twiddle.
 // SCALE: fused n2b pre-scale / b2n post-scale (only the top pass is first / last).
@@ -410,23 +418,36 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
   const u32 l0 = lblk << LOG_C;
   // the next column's elements are loaded while the current one is transformed
   u64 nxt[VALS];
+  // top pass over a column stored bit-reversed: natural row r of column l sits at
+  // bitrev(l) 2^S + bitrev_S(r).  Element e = t + 256 k of the tile is row t mod 2^S of column
+  // l0 + (t >> S) + k 2^(8-S); those three terms have disjoint bits (l0 a multiple of 2^(12-S)),
+  // so bitrev_P0 of the column splits the same way, and the address is the column base plus a
+  // per-thread byte offset that no column changes plus a uniform one per k.  Each thread reads
+  // natural rows in order (permuted reads inside 2^S-element chunks); they go to consecutive
+  // padded LDS rows.
+  // Loads and stores take the column base in SGPRs and a 32-bit lane offset formed from one
+  // register per column: the empty asm makes that register a new value in each iteration, so
+  // LLVM neither hoists 16 zero-extended 64-bit offsets out of the column loop (which costs a
+  // 64-bit add per access instead of the saddr form) nor keeps 16 64-bit bases in SGPRs (which
+  // spill).
+  const u32 gather_off =
+      gather ? ((bitrev32(t >> S, P0) << S) | bitrev32(t & ((1u << S) - 1), S)) * 8u : 0u;
   auto fetch = [&](u32 col) {
-    const u64 *in = col_in(a, col);
+    const u64 *in = in_sgprs(col_in(a, col));
     if constexpr (gather) {
-      // top pass over a column stored bit-reversed: natural row r of column l sits at
-      // bitrev(l) 2^S + bitrev_S(r).  Each thread reads natural rows in order (permuted reads
-      // inside 2^S-element chunks); they go to consecutive padded LDS rows.
+      static_assert(S <= 8, "the gathered pass reads 2^(8-S) columns per thread step");
+      u32 o = gather_off + (bitrev32(l0, P0) << (S + 3));
+      asm volatile("" : "+v"(o));
       static_for<VALS>([&](auto k_) {
         constexpr int k = decltype(k_)::value;
-        const u32 e = t + THREADS * k;
-        const u32 cl = e >> S, row = e & ((1u << S) - 1);
-        const u64 *chunk = in + ((u64)bitrev32(l0 + cl, P0) << S);
-        nxt[k] = gload(chunk + bitrev32(row, S));
+        nxt[k] = gload(at(in, o + (bitrev32((u32)k << (8 - S), P0) << (S + 3))));
       });
     } else {
+      u32 o = off0;
+      asm volatile("" : "+v"(o));
       static_for<VALS>([&](auto m_) {
         constexpr int m = decltype(m_)::value;
-        nxt[m] = gload(at(in + ((u64)m << (WB0 + P0)), off0));
+        nxt[m] = gload(at(in, o + ((u32)m << (WB0 + P0 + 3))));
       });
     }
   };
@@ -434,7 +455,7 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
   const u32 col_end = min(col + a.reps, a.num_cols);
   if (col < col_end) fetch(col);
   for (; col < col_end; col++) {
-    u64 *out = col_out(a, col);
+    u64 *out = in_sgprs(col_out(a, col));
     u64 v[VALS];
     if constexpr (gather) {
       auto stage = [&](auto reduce_) {
@@ -489,10 +510,11 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
       do_round<Sh::K(r), N2B, INV, true>(v, tw[r], !(TOP && r == Sh::TOP));
     });
     if constexpr (!N2B && SCALE) mul_all<true>(v, [&](int m) { return fac[m]; });
+    u32 o = offL;
+    asm volatile("" : "+v"(o));
     static_for<VALS>([&](auto m_) {
       constexpr int m = decltype(m_)::value;
-      u64 *bm = out + ((u64)m << (WBL + P0));
-      gstore(at(bm, offL), v[m]);
+      gstore(at(out, o + ((u32)m << (WBL + P0 + 3))), v[m]);
     });
   }
 }
