@@ -383,7 +383,9 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
   static_assert(S >= 4 && S <= 10, "strided passes cover 4..10 bits");
   constexpr int C = 1 << LOG_C;
   constexpr int ROW = C + 1;           // padded LDS row (odd stride in u64)
-  __shared__ u64 lds[(1 << S) * ROW];
+  // the gathered pass stages row R at R ROW + (R >> GPAD) (S > 6, see below)
+  constexpr int GPAD = GATHER && S > 6 ? S - 6 : -1;
+  __shared__ u64 lds[(1 << S) * ROW + (GPAD >= 0 ? 64 : 0)];
   const u32 t = threadIdx.x;
   const u32 L = a.log_n, P0 = a.lo_bit;
   const u32 tc = bid & ((1u << a.log_tile_stride) - 1);
@@ -419,19 +421,23 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
   // the next column's elements are loaded while the current one is transformed
   u64 nxt[VALS];
   // top pass over a column stored bit-reversed: natural row r of column l sits at
-  // bitrev(l) 2^S + bitrev_S(r).  Element e = t + 256 k of the tile is row t mod 2^S of column
-  // l0 + (t >> S) + k 2^(8-S); those three terms have disjoint bits (l0 a multiple of 2^(12-S)),
-  // so bitrev_P0 of the column splits the same way, and the address is the column base plus a
-  // per-thread byte offset that no column changes plus a uniform one per k.  Each thread reads
-  // natural rows in order (permuted reads inside 2^S-element chunks); they go to consecutive
-  // padded LDS rows.
+  // bitrev(l) 2^S + bitrev_S(r), so column l is one contiguous 2^S-element chunk.  Element
+  // e = t + 256 k of the tile is chunk position t mod 2^S (natural row bitrev_S(t mod 2^S)) of
+  // column l0 + (t >> S) + k 2^(8-S); those three terms have disjoint bits (l0 a multiple of
+  // 2^(12-S)), so bitrev_P0 of the column splits the same way, and the address is the column
+  // base plus a per-thread byte offset that no column changes plus a uniform one per k.  A
+  // wave reads consecutive addresses; the rows go to LDS in natural order.  For S > 6 a wave's
+  // 64 rows are 2^(S-6) apart, which with the odd row stride alone would put them in 8 (S = 8)
+  // or 16 (S = 7) of the 32 eight-byte LDS bank pairs: row R is staged at R ROW + (R >> (S-6)),
+  // which spreads them over all 32.  The first round reads rows (m << (S-4)) | q, whose extra
+  // term (q >> (S-6)) + 4 m stays linear in m.
   // Loads and stores take the column base in SGPRs and a 32-bit lane offset formed from one
   // register per column: the empty asm makes that register a new value in each iteration, so
   // LLVM neither hoists 16 zero-extended 64-bit offsets out of the column loop (which costs a
   // 64-bit add per access instead of the saddr form) nor keeps 16 64-bit bases in SGPRs (which
   // spill).
   const u32 gather_off =
-      gather ? ((bitrev32(t >> S, P0) << S) | bitrev32(t & ((1u << S) - 1), S)) * 8u : 0u;
+      gather ? ((bitrev32(t >> S, P0) << S) | (t & ((1u << S) - 1))) * 8u : 0u;
   auto fetch = [&](u32 col) {
     const u64 *in = in_sgprs(col_in(a, col));
     if constexpr (gather) {
@@ -458,11 +464,12 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
     u64 *out = in_sgprs(col_out(a, col));
     u64 v[VALS];
     if constexpr (gather) {
+      const u32 row = bitrev32(t & ((1u << S) - 1), S);
+      const u32 st0 = row * ROW + (GPAD >= 0 ? row >> GPAD : 0) + (t >> S);
       auto stage = [&](auto reduce_) {
         static_for<VALS>([&](auto k_) {
           constexpr int k = decltype(k_)::value;
-          const u32 e = t + THREADS * k;
-          lds[(e & ((1u << S) - 1)) * ROW + (e >> S)] = decltype(reduce_)::value ? canon(nxt[k]) : nxt[k];
+          lds[st0 + (k << (8 - S))] = decltype(reduce_)::value ? canon(nxt[k]) : nxt[k];
         });
       };
       if (a.in_canonical)
@@ -470,9 +477,11 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
       else
         stage(std::true_type{});
       __syncthreads();
+      static_assert(GPAD < 0 || WB0 == GPAD + 2, "the staged rows' pad is linear in m");
+      const u32 ld0 = lds0 + (GPAD >= 0 ? q >> GPAD : 0);
       static_for<VALS>([&](auto m_) {
         constexpr int m = decltype(m_)::value;
-        v[m] = lds[lds0 + m * (ROW << WB0)];
+        v[m] = lds[ld0 + m * ((ROW << WB0) + (GPAD >= 0 ? 4 : 0))];
       });
       __syncthreads();
     } else {
