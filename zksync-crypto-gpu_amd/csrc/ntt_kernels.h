@@ -418,7 +418,7 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
   const u32 lds0 = window_a<WB0>(q, 0) * ROW + c;
   constexpr bool gather = GATHER;
   const u32 l0 = lblk << LOG_C;
-  // the next column's elements are loaded while the current one is transformed
+  // the next column's elements are loaded while the current one is transformed (PREFETCH below)
   u64 nxt[VALS];
   // top pass over a column stored bit-reversed: natural row r of column l sits at
   // bitrev(l) 2^S + bitrev_S(r), so column l is one contiguous 2^S-element chunk.  Element
@@ -459,9 +459,14 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
   };
   u32 col = grp * a.reps;
   const u32 col_end = min(col + a.reps, a.num_cols);
-  if (col < col_end) fetch(col);
+  // The gathered pass of 6..8 bits loads each column when its turn comes instead of while the
+  // previous one is transformed: without the 16 prefetch registers it fits 128 VGPRs, four waves
+  // per SIMD instead of three (the 4- and 5-bit ones, closer to the HBM bound, keep the prefetch).
+  constexpr bool PREFETCH = !GATHER || S < 6;
+  if (PREFETCH && col < col_end) fetch(col);
   for (; col < col_end; col++) {
     u64 *out = in_sgprs(col_out(a, col));
+    if (!PREFETCH) fetch(col);
     u64 v[VALS];
     if constexpr (gather) {
       const u32 row = bitrev32(t & ((1u << S) - 1), S);
@@ -497,7 +502,7 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
         });
       }
     }
-    if (col + 1 < col_end) fetch(col + 1);
+    if (PREFETCH && col + 1 < col_end) fetch(col + 1);
     if constexpr (N2B && SCALE) mul_all<true>(v, [&](int m) { return fac[m]; });
     static_for<R>([&](auto r_) {
       constexpr int r = decltype(r_)::value;
