@@ -459,10 +459,12 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
   };
   u32 col = grp * a.reps;
   const u32 col_end = min(col + a.reps, a.num_cols);
-  // The gathered pass of 6..8 bits loads each column when its turn comes instead of while the
-  // previous one is transformed: without the 16 prefetch registers it fits 128 VGPRs, four waves
-  // per SIMD instead of three (the 4- and 5-bit ones, closer to the HBM bound, keep the prefetch).
-  constexpr bool PREFETCH = !GATHER || S < 6;
+  // Passes whose 16 prefetch registers would push them past 128 VGPRs (the gathered pass of 6..8
+  // bits, plain passes of 6..8 bits below the top and of 9..10 bits at the top) load each column
+  // when its turn comes instead of while the previous one is transformed, and run four waves per
+  // SIMD instead of three (the 4- and 5-bit gathered ones, nearer the HBM bound, keep the
+  // prefetch).
+  constexpr bool PREFETCH = GATHER ? S < 6 : !((!TOP && S >= 6 && S <= 8) || (TOP && S >= 9));
   if (PREFETCH && col < col_end) fetch(col);
   for (; col < col_end; col++) {
     u64 *out = in_sgprs(col_out(a, col));
