@@ -106,11 +106,13 @@ struct BaPassLaunch {
 //   BA_LAT_AUTO   that, or else the batch holds at most 2^BA_LAT_MAX_LOG_ELEMS elements and the
 //                 tuned tile_log is not 12 (single batch NTT calls).
 enum BaLatPolicy { BA_LAT_NEVER = 0, BA_LAT_TUNED = 1, BA_LAT_AUTO = 2 };
-// Appends the pass launches of one batch NTT (arguments validated) to `launches`.
+// Appends the pass launches of one batch NTT (arguments validated) to `launches`.  shared: the
+// launches will run beside other streams' work (serve batches), so strided passes whose default
+// build runs four waves per SIMD, filling a CU's LDS, take their three-waves build.
 int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint32_t log_n,
                  uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs,
                  bool inverse, uint32_t log_ext, uint32_t coset_idx, bool in_bitrev_gather,
-                 std::vector<BaPassLaunch> &launches, BaLatPolicy lat = BA_LAT_NEVER);
+                 std::vector<BaPassLaunch> &launches, BaLatPolicy lat = BA_LAT_NEVER, bool shared = false);
 // Per-call time over log_n 10..18 and batches of 2^14..2^22 elements (MI355X): up to 2^20
 // elements the latency passes take 1.0-3.1x less time per call, above it the throughput passes win.
 constexpr uint32_t BA_LAT_MAX_LOG_ELEMS = 20;
@@ -148,7 +150,9 @@ struct BaPlanImage {
   std::vector<uint32_t> prefix;                  // first workgroup of each PassArgs in its group
   size_t table_at = 0, prefix_at = 0, bytes = 0;
 };
-int ba_plan_image_build(const ba_context *ctx, const ba_job *jobs, uint32_t num_jobs, BaPlanImage &img);
+// shared: the plan's launches run beside other streams' work (serve batches; see ba_build_ntt)
+int ba_plan_image_build(const ba_context *ctx, const ba_job *jobs, uint32_t num_jobs, BaPlanImage &img,
+                        bool shared = false);
 // Serialises the image into `host` (img.bytes bytes) for a copy to device address `device`.
 void ba_plan_image_write(const BaPlanImage &img, void *host, void *device);
 // Enqueues one group of an image uploaded at `device`.

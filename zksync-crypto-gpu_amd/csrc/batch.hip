@@ -89,11 +89,11 @@ struct PendingLaunch {
 int add_transform(const ba_context *ctx, uint32_t log_n, bool bitrev, bool inverse, uint32_t log_ext,
                   uint32_t coset, bool gather, BaLatPolicy lat, const std::vector<uint64_t *> &in,
                   const std::vector<uint64_t *> &out, uint32_t first_level, std::vector<const void *> &table,
-                  std::vector<PendingLaunch> &pending, uint32_t *depth) {
+                  std::vector<PendingLaunch> &pending, uint32_t *depth, bool shared) {
   std::vector<BaPassLaunch> ls;
   // merged NTT jobs take the latency-shaped passes (grouped like the others) where tuned to
   const int rc = ba_build_ntt(ctx, nullptr, nullptr, log_n, (uint32_t)in.size(), 0, 0, bitrev, inverse, log_ext,
-                              coset, gather, ls, lat);
+                              coset, gather, ls, lat, shared);
   if (rc) return rc;
   const size_t li = table.size();
   table.insert(table.end(), in.begin(), in.end());
@@ -111,7 +111,8 @@ int add_transform(const ba_context *ctx, uint32_t log_n, bool bitrev, bool inver
 
 }  // namespace
 
-int ba_plan_image_build(const ba_context *ctx, const ba_job *jobs, uint32_t num_jobs, BaPlanImage &img) {
+int ba_plan_image_build(const ba_context *ctx, const ba_job *jobs, uint32_t num_jobs, BaPlanImage &img,
+                        bool shared) {
   img = BaPlanImage{};
   if (num_jobs && !jobs) return BA_ERROR_INVALID_VALUE;
   // 1. columns of all jobs, merged by signature
@@ -150,15 +151,15 @@ int ba_plan_image_build(const ba_context *ctx, const ba_job *jobs, uint32_t num_
     int rc;
     if (sg.kind == BA_JOB_NTT) {
       rc = add_transform(ctx, sg.log_n, sg.bitrev, sg.inverse, sg.log_ext, sg.coset, false, BA_LAT_TUNED, c.in, c.out, 0,
-                         img.table, pending, nullptr);
+                         img.table, pending, nullptr, shared);
     } else {
       // n2b inverse NTT to bit-reversed monomials, then every coset's forward NTT gathers them
       uint32_t depth = 0;
       rc = add_transform(ctx, sg.log_n, false, true, 0, 0, false, BA_LAT_NEVER, c.in, c.mono, 0, img.table, pending,
-                         &depth);
+                         &depth, shared);
       for (uint32_t cs = 0; rc == BA_SUCCESS && cs < c.coset.size(); cs++)
         rc = add_transform(ctx, sg.log_n, false, false, sg.log_blowup, bitrev_bits(cs, sg.log_blowup), true, BA_LAT_NEVER,
-                           c.mono, c.coset[cs], depth, img.table, pending, nullptr);
+                           c.mono, c.coset[cs], depth, img.table, pending, nullptr, shared);
     }
     if (rc) return rc;
   }

@@ -86,21 +86,22 @@ using namespace gl::ntt;
 // scale: 0 none, 1 one factor for every element, 2 per-element coset factors (the strided
 // kernels fuse 1 and 2 alike)
 // waves: the 9..12-bit contiguous pass's variant (3 or 4 waves per SIMD)
+// shared: the launch shares the GPU with other streams' work (strided passes: three-waves builds)
 BaKernels pick_kernel(bool contig, uint32_t S, bool n2b, bool inverse, bool top, int scale,
-                      bool whole, bool gather, uint32_t waves) {
+                      bool whole, bool gather, uint32_t waves, bool shared) {
   if (contig) {
     if (S <= 8) return ba_pick_contig_lo(S, n2b, inverse, scale, whole);
     return waves == 3 ? ba_pick_contig_hi3(S, n2b, inverse, scale, whole)
                       : ba_pick_contig_hi(S, n2b, inverse, scale, whole);
   }
-  return ba_pick_strided(S, n2b, inverse, top, scale != 0, gather);
+  return ba_pick_strided(S, n2b, inverse, top, scale != 0, gather, shared);
 }
 
 // kernel names as rocprofv3 prints them, for launch tracing; the table is built once, by the
 // first caller (a function-local static initialiser, so concurrent first calls are safe)
 struct KernelNames {
-  // [contig][S][n2b][inverse][top or whole][scale][gather or 4-waves]
-  char n[2][13][2][2][2][3][2][120];
+  // [strided, contig, strided shared build][S][n2b][inverse][top or whole][scale][gather or 4-waves]
+  char n[3][13][2][2][2][3][2][120];
   KernelNames() {
     static const char *tf[2] = {"false", "true"};
     for (int s = 0; s < 13; s++)
@@ -115,14 +116,18 @@ struct KernelNames {
                 snprintf(n[0][s][d][i][tp][sc][ga], sizeof(n[0][s][d][i][tp][sc][ga]),
                          "void gl::ntt::ntt_pass_strided<%d, %s, %s, %s, %s, %s>(gl::ntt::PassArgs)", s,
                          tf[d], tf[i], tf[tp], tf[sc != 0], tf[ga]);
+                snprintf(n[2][s][d][i][tp][sc][ga], sizeof(n[2][s][d][i][tp][sc][ga]),
+                         "void gl::ntt::ntt_pass_strided_shared<%d, %s, %s, %s, %s, %s>(gl::ntt::PassArgs)", s,
+                         tf[d], tf[i], tf[tp], tf[sc != 0], tf[ga]);
               }
   }
 };
 
+// shared_build: a strided pass's three-waves build for shared launches
 const char *kernel_name(bool contig, uint32_t S, bool n2b, bool inverse, bool top, int scale,
-                        bool gather, uint32_t waves) {
+                        bool gather, uint32_t waves, bool shared_build) {
   static const KernelNames names;
-  return names.n[contig][S][n2b][inverse][top][scale][contig ? waves == 4 : gather];
+  return names.n[contig ? 1 : (shared_build ? 2 : 0)][S][n2b][inverse][top][scale][contig ? waves == 4 : gather];
 }
 
 constexpr uint64_t kMaxGrid = 0x7FFFFFFFull;
@@ -178,7 +183,7 @@ bool ba_use_latency_passes(const ba_context *ctx, uint32_t log_n, bool n2b, uint
 int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint32_t log_n,
                  uint32_t num_cols, uint64_t stride_in, uint64_t stride_out, bool bitrev_inputs,
                  bool inverse, uint32_t log_ext, uint32_t coset_idx, bool in_bitrev_gather,
-                 std::vector<BaPassLaunch> &launches, BaLatPolicy lat) {
+                 std::vector<BaPassLaunch> &launches, BaLatPolicy lat, bool shared) {
   if (num_cols == 0) return BA_SUCCESS;
   const bool n2b = !bitrev_inputs;
   PassArgs base{};
@@ -304,8 +309,10 @@ int ba_build_ntt(const ba_context *ctx, const uint64_t *in, uint64_t *out, uint3
       uint32_t waves = 3;
       if (contig && S >= 9)
         waves = tuning.valid && tuning.contig_waves ? tuning.contig_waves : (l.grid < 2048 ? 3 : 4);
-      l.kernels = pick_kernel(contig, S, n2b, inverse, top, scale, big, gather, waves);
-      l.name = kernel_name(contig, S, n2b, inverse, contig ? big : top, scale, gather, waves);
+      l.kernels = pick_kernel(contig, S, n2b, inverse, top, scale, big, gather, waves, shared);
+      const bool shared_build =
+          shared && !contig && l.kernels.single != pick_kernel(contig, S, n2b, inverse, top, scale, big, gather, waves, false).single;
+      l.name = kernel_name(contig, S, n2b, inverse, contig ? big : top, scale, gather, waves, shared_build);
       l.pass = p;
       l.col0 = c0;
       if (!l.kernels.single) return BA_ERROR_INVALID_VALUE;

@@ -22,7 +22,7 @@ BaKernels ba_pick_contig_hi(uint32_t S, bool n2b, bool inverse, int scale, bool 
 BaKernels ba_pick_contig_hi3(uint32_t S, bool n2b, bool inverse, int scale, bool whole);  // 3, prefetch
 // strided pass, 4 <= S <= 10; top: the pass holds the top bits; scale: fused (pre/post) scale;
 // gather: the n2b top pass reads bit-reversed columns
-BaKernels ba_pick_strided(uint32_t S, bool n2b, bool inverse, bool top, bool scale, bool gather);
+BaKernels ba_pick_strided(uint32_t S, bool n2b, bool inverse, bool top, bool scale, bool gather, bool shared);
 // whole transform of log_n <= 3 per thread
 BaKernels ba_pick_tiny(uint32_t log_n, bool n2b, bool inverse);
 // latency-shaped passes (ntt_lat.h), 1024-element tiles: contiguous over 4..10 bits, strided over

@@ -373,7 +373,19 @@ __device__ __forceinline__ T *in_sgprs(T *p) {
 // TOP: the pass holds the index's top bits (P0 + S == L), so one round has no twiddle.
 // SCALE: fused n2b pre-scale / b2n post-scale (only the top pass is first / last).
 // GATHER: the n2b top pass reads a column stored bit-reversed.
-template <int S, bool N2B, bool INV, bool TOP, bool SCALE, bool GATHER>
+// Strided variants whose next-column prefetch (16 more registers) would push them past 128 VGPRs:
+// the gathered pass of 6..8 bits, plain passes of 6..8 bits below the top and of 9..10 bits at the
+// top.  Their default build loads each column when its turn comes instead of while the previous
+// one is transformed and runs four waves per SIMD instead of three (the 4- and 5-bit gathered
+// ones, nearer the HBM bound, keep the prefetch).  Four-waves workgroups fill a CU's LDS, so a
+// launch that shares the GPU with other streams' work (serve batches) takes the prefetching
+// three-waves build, which leaves room for theirs.
+template <int S, bool TOP, bool GATHER>
+constexpr bool strided_four_waves() {
+  return GATHER ? S >= 6 : ((!TOP && S >= 6 && S <= 8) || (TOP && S >= 9));
+}
+
+template <int S, bool N2B, bool INV, bool TOP, bool SCALE, bool GATHER, bool PREFETCH>
 __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
   static_assert(TOP || !SCALE, "only the top pass scales");
   static_assert(!GATHER || (N2B && TOP), "only the n2b top pass gathers");
@@ -459,12 +471,6 @@ __device__ __forceinline__ void strided_body(const PassArgs &a, const u32 bid) {
   };
   u32 col = grp * a.reps;
   const u32 col_end = min(col + a.reps, a.num_cols);
-  // Passes whose 16 prefetch registers would push them past 128 VGPRs (the gathered pass of 6..8
-  // bits, plain passes of 6..8 bits below the top and of 9..10 bits at the top) load each column
-  // when its turn comes instead of while the previous one is transformed, and run four waves per
-  // SIMD instead of three (the 4- and 5-bit gathered ones, nearer the HBM bound, keep the
-  // prefetch).
-  constexpr bool PREFETCH = GATHER ? S < 6 : !((!TOP && S >= 6 && S <= 8) || (TOP && S >= 9));
   if (PREFETCH && col < col_end) fetch(col);
   for (; col < col_end; col++) {
     u64 *out = in_sgprs(col_out(a, col));
@@ -811,24 +817,40 @@ __device__ __forceinline__ u32 group_of(const u32 *wg_start, u32 njobs, u32 b) {
 #ifndef BA_XCD_REMAP_MAX_S
 #define BA_XCD_REMAP_MAX_S 4
 #endif
-template <int S, bool N2B, bool INV, bool TOP, bool SCALE, bool GATHER>
-__global__ __launch_bounds__(THREADS) void ntt_pass_strided(PassArgs a) {
+template <int S>
+__device__ __forceinline__ u32 strided_tile() {
   if constexpr (S <= BA_XCD_REMAP_MAX_S || S >= 9) {
     // Workgroups are dealt round-robin over the 8 XCDs, so neighbouring tiles (neighbouring runs
     // of the same 2^S rows, 2^(log_n - S) elements apart) would run at once on all eight and
     // crowd the same HBM channels.  XCD x instead takes the contiguous tile range starting at
     // x q + min(x, r), q or q + 1 tiles (q = G / 8, r = G % 8): a bijection on [0, G).
     const u32 G = gridDim.x, b = blockIdx.x, x = b & 7u, q = G >> 3, r = G & 7u;
-    strided_body<S, N2B, INV, TOP, SCALE, GATHER>(a, x * q + (x < r ? x : r) + (b >> 3));
+    return x * q + (x < r ? x : r) + (b >> 3);
   } else {
-    strided_body<S, N2B, INV, TOP, SCALE, GATHER>(a, blockIdx.x);
+    return blockIdx.x;
   }
+}
+template <int S, bool N2B, bool INV, bool TOP, bool SCALE, bool GATHER>
+__global__ __launch_bounds__(THREADS) void ntt_pass_strided(PassArgs a) {
+  strided_body<S, N2B, INV, TOP, SCALE, GATHER, !strided_four_waves<S, TOP, GATHER>()>(a, strided_tile<S>());
 }
 template <int S, bool N2B, bool INV, bool TOP, bool SCALE, bool GATHER>
 __global__ __launch_bounds__(THREADS) void ntt_pass_strided_group(const PassArgs *jobs, const u32 *wg_start,
                                                                   u32 njobs) {
   const u32 j = group_of(wg_start, njobs, blockIdx.x);
-  strided_body<S, N2B, INV, TOP, SCALE, GATHER>(jobs[j], blockIdx.x - wg_start[j]);
+  strided_body<S, N2B, INV, TOP, SCALE, GATHER, !strided_four_waves<S, TOP, GATHER>()>(jobs[j],
+                                                                                       blockIdx.x - wg_start[j]);
+}
+// the prefetching three-waves build of a four-waves variant, for launches that share the GPU
+template <int S, bool N2B, bool INV, bool TOP, bool SCALE, bool GATHER>
+__global__ __launch_bounds__(THREADS) void ntt_pass_strided_shared(PassArgs a) {
+  strided_body<S, N2B, INV, TOP, SCALE, GATHER, true>(a, strided_tile<S>());
+}
+template <int S, bool N2B, bool INV, bool TOP, bool SCALE, bool GATHER>
+__global__ __launch_bounds__(THREADS) void ntt_pass_strided_shared_group(const PassArgs *jobs,
+                                                                         const u32 *wg_start, u32 njobs) {
+  const u32 j = group_of(wg_start, njobs, blockIdx.x);
+  strided_body<S, N2B, INV, TOP, SCALE, GATHER, true>(jobs[j], blockIdx.x - wg_start[j]);
 }
 // W: waves per SIMD the contiguous pass is compiled for.  4: no prefetch, 128 registers (fastest
 // per launch over many tiles); 3: with the next-tile prefetch (fewer instructions per tile, better

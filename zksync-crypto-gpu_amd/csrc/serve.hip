@@ -237,7 +237,8 @@ bool ba_server::launch(Slot &s, std::unique_lock<std::mutex> &lk) {
   while (count) {  // the largest prefix of the valid requests whose plan fits the slot
     jobs.clear();
     for (size_t k = 0; k < count; k++) jobs.push_back(request_job(rs[good[k]], base));
-    rc = ba_plan_image_build(ctx, jobs.data(), (uint32_t)jobs.size(), img);
+    // batches run on several slot streams at once: three-waves strided builds leave them room
+    rc = ba_plan_image_build(ctx, jobs.data(), (uint32_t)jobs.size(), img, true);
     if (rc || img.bytes <= kDescBytes || count == 1) break;
     count = (count + 1) / 2;
   }
